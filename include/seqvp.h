@@ -1,0 +1,169 @@
+/* C-ABI of libseqvp_hip.so: the MI355X (gfx950) kernels of the seq_vp_3d registration hot path.
+ *
+ * Conventions (every entry point):
+ *   - plain device pointers and sizes, no torch types; the caller owns every buffer (kernels never
+ *     allocate or free); `stream` is a hipStream_t passed as void* (NULL = the null stream);
+ *   - node fields are (nz, ny, nx, 3) float64, components xyz, x fastest (jd_curl layout);
+ *     free-node vectors are component-major (3, nz-2, ny-2, nx-2);
+ *   - `k` points to the 39-double Q1 constant vector in device memory:
+ *     k[0:3]=1/h (xyz), k[3:6]=Gauss nodes, k[6:33]=Gauss weights omega*wx*wy*wz (point 9iz+3iy+ix),
+ *     k[33]=mu, k[34]=nu, k[35]=barrier weight kappa*omega/27, k[36]=delta, k[37]=activation,
+ *     k[38]=activation-delta;
+ *   - return 0 on success, a negative SEQVP_E* code otherwise; nothing throws across the ABI;
+ *   - all work is enqueued on `stream`; results are valid after that stream is synchronised.
+ *   - the library holds no mutable global state: calls are reentrant across host threads/streams.
+ */
+#ifndef SEQVP_H
+#define SEQVP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEQVP_OK 0
+#define SEQVP_EINVAL -1     /* bad size/shape/pointer */
+#define SEQVP_ELAUNCH -2    /* kernel launch failed */
+#define SEQVP_EWORKSPACE -3 /* workspace too small */
+
+/* Version string of the library build. */
+const char* seqvp_version(void);
+
+/* Workspace bytes an entry point needs for a grid of nx*ny*nz nodes.
+ * op: 0 soft energy, 1 barrier energy, 2 gradient, 3 GGN diagonal, 4 joint outer gradients,
+ *     5 JD root, 6 DST apply, 7 dot products, 8 node-min JD. */
+size_t seqvp_workspace_bytes(int op, int64_t nx, int64_t ny, int64_t nz);
+
+/* Soft JD/curl change energies (replaces FusedQ1.soft_energy,
+ * jd_curl/three_d/backends/triton_q1.py:572-609; formulas residuals.py:166-172 and
+ * energy_comparison.py:196-200).  out[0] = 1/2 mu sum qw (J-Jr)^2, out[1] = 1/2 nu sum qw |c-cr|^2,
+ * out[2] = sum 1/2 (y-x)(y+x) over the JD/curl residuals of u (y) and base (x); 0 when base == NULL.
+ * Sums are compensated (double-double). `out` is 3 device doubles. */
+int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base, const double* k,
+                         int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
+/* Closed27 localized barrier energy (replaces FusedQ1.barrier_energy, triton_q1.py:680-719;
+ * barrier.py:19-36, energy_comparison.py:14-46).  out[0] = w sum psi(J), out[1] = min J,
+ * out[2] = UNWEIGHTED sum of stable psi(J)-psi(Ja) vs base (the caller multiplies by w, as
+ * accelerated.py:1627 does), out[3] = min base J (0 / +inf when base == NULL).
+ * A NaN J propagates to out[1]. `out` is 4 device doubles. */
+int seqvp_q1_barrier_energy(const double* u, const double* base, const double* k, int64_t nx, int64_t ny,
+                            int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Node-term energies (replaces the image/proximity part of AcceleratedHybridBackend._energy,
+ * accelerated.py:1583-1600; residuals.py:189-195, energy_comparison.py:196-210).
+ * q is the trapezoid node mass omega*tz*ty*tx computed in-kernel; qd = q*data_weight (data_weight may
+ * be NULL = 1).  out[0] = 1/2 sum qd (W-F)^2, out[1] = 1/2 lam sum q |u-ref|^2, and with a base
+ * (base_warped and base_u non-NULL): out[2] = sum 1/2 (y-x)(y+x) of the image residuals (outer F
+ * block), out[3] = the same over the proximity residuals. `out` is 4 device doubles. */
+int seqvp_node_energy(const double* warped, const double* fixed, const double* data_weight, const double* u,
+                      const double* ref, const double* base_warped, const double* base_u, double lam,
+                      double hx, double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* out,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* Energy gradient on the nodes (replaces FusedQ1.gradient_nodes, triton_q1.py:504-518;
+ * linearization.py:159-201): out = node_base + [Gauss JD/curl VJP] + [closed27 barrier VJP].
+ * node_base may be NULL (zero). out may alias node_base. out is (nz,ny,nx,3). */
+int seqvp_q1_gradient(const double* node_base, const double* u, const double* ref, int soft, int barrier,
+                      const double* k, int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* GF and GE node gradients sharing the cell work (replaces FusedQ1.joint, triton_q1.py:563-570;
+ * linearization.py:217-238): outer = outer_base + barrier VJP; total = total_base + soft VJP +
+ * barrier VJP. */
+int seqvp_q1_joint(const double* outer_base, const double* total_base, const double* u, const double* ref,
+                   const double* k, int64_t nx, int64_t ny, int64_t nz, double* outer, double* total,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Exact GGN diagonal on the nodes (replaces FusedQ1.diagonal_nodes, triton_q1.py:442-458;
+ * linearization.py:357-380): out = node_base + per-cell Gauss soft + closed27 barrier diagonal. */
+int seqvp_q1_ggn_diagonal(const double* node_base, const double* u, const double* k, int64_t nx, int64_t ny,
+                          int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* T62 node JD measure without the T69 extension (replaces the closed27/edge part of
+ * TorchLinearization.node_min_closed27_jd, torch_ops.py:687-773; jd_affine.py:24-48,205-225):
+ * node[n] = min over the cells of n of min(closed27 J, exact edge-quadratic minima); +inf if none. */
+int seqvp_node_min_jd(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz, double* node,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* T66 global closed27 first JD root (replaces FusedJDRoot.__call__, triton_jd_root.py:38-52;
+ * global_jd_step.py:22-129). u: base field, v: direction field (both node fields), limit > 0.
+ * out[0] = min root in s-units (inf if none), out[1] = number of points with c0 <= 0 (as double),
+ * out[2] = 1.0 if any cubic coefficient was not finite, else 0.0. `out` is 3 device doubles. */
+int seqvp_jd_root(const double* u, const double* v, const double* k, double limit, int64_t nx, int64_t ny,
+                  int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* T75 Bernstein floating-point prescreen (replaces prescreen_lower_bounds,
+ * jd_curl/three_d/certificate_prescreen.py:125-153; kernel :60-121), bitwise in IEEE binary64 with no
+ * contraction and the reference operation order. D: closed27 derivative table (3,27,8) (device).
+ * cells: increasing int64 cell ids (device) or NULL for all cells (then ncells must equal the cell
+ * count). out[i] = min beta_fl - B for cell i, NaN where a value is not finite. */
+int seqvp_prescreen(const double* u, const double* D, const int64_t* cells, int64_t ncells, int64_t nx,
+                    int64_t ny, int64_t nz, double* out, void* stream);
+
+/* Prescreen of all cells fused with the proof test: flags[c] = 1 if min beta_fl - B > delta (proven),
+ * else 0; out_count[0] = number of unproven cells (int64, device). Same arithmetic as seqvp_prescreen. */
+int seqvp_prescreen_flags(const double* u, const double* D, double delta, int64_t nx, int64_t ny, int64_t nz,
+                          uint8_t* flags, int64_t* out_count, void* stream);
+
+/* Tensor cubic B-spline value and gradient at node positions X + u (replaces TorchSpline.sample /
+ * sample_points, jd_curl/three_d/backends/spline.py:81-127; interpolation.py:72-113).
+ * coeff: (cz, cy, cx) coefficients; tz/ty/tx: knot vectors of lengths ntz/nty/ntx (cubic, so
+ * coefficient counts are nt-4). origin/spacing: xyz of the node grid (nx,ny,nz) whose positions
+ * are sampled. value: (N) or NULL; grad: (N,3) xyz or NULL. status[0] (int64, device) counts queries
+ * outside the closed support, or, for free nodes, outside the open support (0 = all valid). */
+int seqvp_spline_warp(const double* u, const double* coeff, int64_t cz, int64_t cy, int64_t cx,
+                      const double* tz, int64_t ntz, const double* ty, int64_t nty, const double* tx, int64_t ntx,
+                      double ox, double oy, double oz, double hx, double hy, double hz, int64_t nx, int64_t ny,
+                      int64_t nz, double* value, double* grad, int64_t* status, void* stream);
+
+/* Same sampler at arbitrary points (npts, 3) xyz; status[0] counts points outside the closed box
+ * [lox,hix]x[loy,hiy]x[loz,hiz] (their value/gradient are NaN). */
+int seqvp_spline_points(const double* points, int64_t npts, const double* coeff, int64_t cz, int64_t cy,
+                        int64_t cx, const double* tz, int64_t ntz, const double* ty, int64_t nty,
+                        const double* tx, int64_t ntx, double lox, double hix, double loy, double hiy,
+                        double loz, double hiz, double* value, double* grad, int64_t* status, void* stream);
+
+/* DST-I Poisson inverse (replaces TorchPoisson.apply, torch_ops.py:107-115 and
+ * _HostPoisson.apply_equilibrated, hybrid.py:262-286; PoissonInverse3D.apply, poisson.py:99-106):
+ *   out = sign * s * IDST( DST(s * rhs) / den ),  DST = orthonormal type-I along x, y, z,
+ * rhs/out: (3, mz, my, mx) (mz,my,mx = interior sizes), den: (mz, my, mx), s: same layout as rhs or
+ * NULL (= 1), sign = -1 when `negate` else +1. Dense sine-matrix contractions on MFMA (FP64).
+ * out may alias rhs. */
+int seqvp_dst_poisson(const double* rhs, const double* den, const double* scale, int negate, int64_t mx,
+                      int64_t my, int64_t mz, int64_t ncomp, double* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* FP32 variant for the precision form (replaces TorchPoisson.apply_fp32, torch_ops.py:117-136):
+ * rhs/out float64 in memory, transforms and division in float32 on MFMA. */
+int seqvp_dst_poisson_f32(const double* rhs, const double* den, const double* scale, int negate, int64_t mx,
+                          int64_t my, int64_t mz, int64_t ncomp, double* out, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/* Compensated dot products of n-vectors: out[i] = sum x_i * y_i for the pairs given (npairs <= 4);
+ * xs/ys are arrays of device pointers in host memory. */
+int seqvp_dots(const double* const* xs, const double* const* ys, int npairs, int64_t n, double* out,
+               void* workspace, size_t workspace_bytes, void* stream);
+
+/* theta (3, nz-2, ny-2, nx-2) -> node field (nz, ny, nx, 3) with a zero boundary (boundary.py:8-12). */
+int seqvp_inject(const double* theta, int64_t nx, int64_t ny, int64_t nz, double* u, void* stream);
+
+/* node field -> theta on the free nodes (boundary.py:15-17; linearization.py:84-88). */
+int seqvp_restrict(const double* u, int64_t nx, int64_t ny, int64_t nz, double* theta, void* stream);
+
+/* Candidate map of the line search, u = inject(theta + tau p) with theta_out = theta + tau p when
+ * theta_out != NULL; the product and sum are rounded separately, as the host axpy does
+ * (poisson_gradient.py:1403-1437). */
+int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t nx, int64_t ny, int64_t nz,
+                      double* theta_out, double* u, void* stream);
+
+/* One v_mfma_f64_16x16x4f64 on A (16x4, row-major a) and B (4x16, row-major b) written to d (16x16,
+ * row-major) through the lane mapping the DST kernels assume; a layout self-check. */
+int seqvp_mfma_probe(const double* a, const double* b, double* d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,107 @@
+"""ctypes binding of libseqvp_hip.so (the C-ABI declared in include/seqvp.h).
+
+The library is built in-tree by `seq_vp_3d_amd.build.build()`. There is no fallback: if the library
+is missing or a call fails, an exception is raised.
+"""
+import ctypes
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().with_name("libseqvp_hip.so")
+
+c_int, c_i64, c_dbl, c_vp, c_sz = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+
+# name -> argument types (all return int status unless listed in _RESTYPES)
+SIGNATURES = {
+    "seqvp_version": [],
+    "seqvp_workspace_bytes": [c_int, c_i64, c_i64, c_i64],
+    "seqvp_q1_soft_energy": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_barrier_energy": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_node_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64,
+                          c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_gradient": [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_joint": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_ggn_diagonal": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_node_min_jd": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_jd_root": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_prescreen": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],
+    "seqvp_prescreen_flags": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_spline_warp": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_dbl, c_dbl,
+                          c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "seqvp_spline_points": [c_vp, c_i64, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_dbl,
+                            c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
+    "seqvp_dst_poisson": [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_dst_poisson_f32": [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_dots": [c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_inject": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
+    "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
+    "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_mfma_probe": [c_vp, c_vp, c_vp, c_vp],
+}
+_RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz}
+
+ERRORS = {-1: "invalid argument", -2: "kernel launch failed", -3: "workspace too small"}
+
+_LIB = None
+
+
+class SeqvpError(RuntimeError):
+    """A C-ABI call returned a nonzero status."""
+
+
+def load():
+    """The loaded library with every declared signature bound (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH.name} is not built; run seq_vp_3d_amd.build.build() first")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, c_int)
+        _LIB = lib
+    return _LIB
+
+
+def call(name, *args):
+    code = getattr(load(), name)(*args)
+    if code != 0:
+        raise SeqvpError(f"{name}: {ERRORS.get(code, code)}")
+
+
+def ptr(t):
+    """Device address of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("tensor expected")
+    if not t.is_cuda:
+        raise ValueError("device tensor expected: the MI355X path has no CPU fallback")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor expected")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def workspace_bytes(op, shape_zyx):
+    nz, ny, nx = shape_zyx
+    return int(load().seqvp_workspace_bytes(op, nx, ny, nz))
+
+
+class Workspace:
+    """One growable device byte buffer shared by the calls of a model on one stream."""
+
+    def __init__(self, device):
+        self.device = device
+        self.buf = None
+
+    def get(self, nbytes):
+        nbytes = max(int(nbytes), 256)
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self.buf, nbytes

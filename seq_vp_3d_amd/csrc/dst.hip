@@ -1,0 +1,279 @@
+// DST-I Poisson inverse on MFMA (gfx950).
+//
+// p = sign * s * IDST( DST(s * g) / den ) with DST the orthonormal type-I transform along x, y and z
+// (poisson.py:99-106; torch_ops.py:98-115; hybrid.py:262-286). The interior lengths of the native
+// grid are 222/190/158 (periods 223, 191 prime, 159 = 3*53), so there is no small-radix FFT; each
+// axis is a dense contraction with the symmetric sine matrix S[k][j] = sqrt(2/(n+1)) sin(pi (j+1)(k+1)/(n+1)),
+// run as batched GEMMs on the matrix cores:
+//   x axis (contiguous):  Out (O x n) = X (O x n) * S            (O = ncomp*mz*my rows)
+//   y/z axes (strided):   Out_o (n x I) = S * X_o (n x I)         (I = inner stride, batch o)
+// The pre-scale s*g is fused into the first pass's loads, the division by the spectrum into the
+// third pass's stores and the post-scale/negation into the last pass's stores: 6 passes over the
+// 3F-vector, ping-ponging through one workspace buffer.
+//
+// Tiles: one workgroup = 4 wave64 computes a 64x64 output tile; each wave owns a 32x32 quadrant as
+// 2x2 v_mfma_f64_16x16x4f64 accumulators. Operand fragments of the data matrix are staged through
+// LDS in 64x16 (or 16x64) k-slabs with coalesced global loads; S fragments are read from L2
+// (S is at most 222^2 doubles).
+#include <math.h>
+
+#include "q1_common.h"
+#include "abi_util.h"
+
+namespace seqvp {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef d4 type; };
+template <> struct Acc<float> { typedef f4 type; };
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// sine matrix of length n into S (n*n), row-major, exact angle reduction m = (j+1)(k+1) mod 2(n+1),
+// evaluated in binary64 and rounded once to T
+template <typename T>
+__global__ void sine_matrix_kernel(T* __restrict__ S, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n * n) return;
+  const int64_t k = i / n, j = i % n;
+  const int64_t period = 2 * (n + 1);
+  const int64_t m = ((j + 1) * (k + 1)) % period;
+  S[i] = (T)(sqrt(2.0 / (double)(n + 1)) * sinpi((double)m / (double)(n + 1)));
+}
+
+// Epilogue/prologue modes.
+enum : int { PRE_NONE = 0, PRE_SCALE = 1 };
+enum : int { POST_NONE = 0, POST_DIV = 1, POST_SCALE = 2 };
+
+// ---- x axis: Out[o][kk] = sum_j X[o][j] S[j][kk]  (M = O rows, N = K = n) --------------------------
+// A = X (row-major, ld n), B = S. Block tile: rows m0..m0+63, cols n0..n0+63. PRE_SCALE forms
+// s*x in binary64 before rounding to T (the equilibration pre-scale); POST_SCALE writes
+// sign*(s*y) in binary64.
+template <typename T, typename TIN, typename TOUT, int PRE, int POST>
+__global__ void __launch_bounds__(256) dst_rows_kernel(const TIN* __restrict__ X, const double* __restrict__ pre,
+                                                       const T* __restrict__ S, int64_t O, int64_t n,
+                                                       const double* __restrict__ post, double sign,
+                                                       TOUT* __restrict__ Y) {
+  static_assert(POST != POST_DIV, "the spectral division is fused into a strided pass");
+  typedef typename Acc<T>::type AT;
+  __shared__ T As[TK][TM + 1];   // As[k][m]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * TM, n0 = (int64_t)blockIdx.y * TN;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  AT acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  for (int64_t k0 = 0; k0 < n; k0 += TK) {
+    // stage A tile: 64 rows x 16 k, coalesced along k (row-major X)
+#pragma unroll
+    for (int t = 0; t < (TM * TK) / 256; ++t) {
+      const int e = t * 256 + threadIdx.x;
+      const int r = e / TK, kk = e % TK;
+      const int64_t gm = m0 + r, gk = k0 + kk;
+      T v = 0;
+      if (gm < O && gk < n) {
+        const int64_t idx = gm * n + gk;
+        if (PRE == PRE_SCALE)
+          v = (T)(pre[idx] * (double)X[idx]);
+        else
+          v = (T)X[idx];
+      }
+      As[kk][r] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < TK; ks += 4) {
+      const int64_t gk = k0 + ks + lk;
+      T bf[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int64_t gn = n0 + wn + 16 * b + li;
+        bf[b] = (gk < n && gn < n) ? S[gk * n + gn] : (T)0;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const T af = As[ks + lk][wm + 16 * a + li];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma(af, bf[b], acc[a][b]);
+      }
+    }
+    __syncthreads();
+  }
+  // D layout: lane holds column li, rows 4*lk + r
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm + 16 * a + 4 * lk + r, gn = n0 + wn + 16 * b + li;
+        if (gm < O && gn < n) {
+          const int64_t idx = gm * n + gn;
+          if (POST == POST_SCALE) {
+            const double v = (double)acc[a][b][r];
+            Y[idx] = (TOUT)(sign * (post ? post[idx] * v : v));
+          } else {
+            Y[idx] = (TOUT)acc[a][b][r];
+          }
+        }
+      }
+}
+
+// ---- strided axes: Out_o[kk][i] = sum_j S[kk][j] X_o[j][i]  (M = n, N = I, K = n, batch o) ----------
+// A = S (L2), B = X_o staged in LDS as Bs[k][i] (coalesced along i). With POST_DIV the result is
+// divided by den[kk*I + i] rounded to T (the z pass: den has the interior shape).
+template <typename T, int POST>
+__global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, const T* __restrict__ S, int64_t n,
+                                                       int64_t I, const double* __restrict__ den,
+                                                       T* __restrict__ Y) {
+  typedef typename Acc<T>::type AT;
+  __shared__ T Bs[TK][TN + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t o = blockIdx.z;
+  const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+  const T* Xo = X + o * n * I;
+  T* Yo = Y + o * n * I;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  AT acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  for (int64_t k0 = 0; k0 < n; k0 += TK) {
+#pragma unroll
+    for (int t = 0; t < (TK * TN) / 256; ++t) {
+      const int e = t * 256 + threadIdx.x;
+      const int kk = e / TN, c = e % TN;
+      const int64_t gk = k0 + kk, gi = n0 + c;
+      Bs[kk][c] = (gk < n && gi < I) ? Xo[gk * I + gi] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < TK; ks += 4) {
+      const int64_t gk = k0 + ks + lk;
+      T af[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int64_t gm = m0 + wm + 16 * a + li;
+        af[a] = (gm < n && gk < n) ? S[gm * n + gk] : (T)0;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const T bf = Bs[ks + lk][wn + 16 * b + li];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) acc[a][b] = mfma(af[a], bf, acc[a][b]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm + 16 * a + 4 * lk + r, gi = n0 + wn + 16 * b + li;
+        if (gm < n && gi < I) {
+          const int64_t idx = gm * I + gi;
+          T v = acc[a][b][r];
+          if (POST == POST_DIV) v = v / (T)den[idx];
+          Yo[idx] = v;
+        }
+      }
+}
+
+// Layout probe: D = A(16x4) * B(4x16) with one MFMA, A[i][k] = a[i*4+k], B[k][j] = b[k*16+j];
+// writes D[i][j] to d[i*16+j] using the documented lane mapping. Used by the tests to confirm the
+// operand/accumulator layout the kernels above assume.
+__global__ void mfma_probe_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ d) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const double af = a[li * 4 + lk];
+  const double bf = b[lk * 16 + li];
+  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc, 0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) d[(4 * lk + r) * 16 + li] = acc[r];
+}
+
+}  // namespace seqvp
+
+using namespace seqvp;
+
+size_t seqvp_dst_workspace_bytes(int64_t mx, int64_t my, int64_t mz, int64_t ncomp) {
+  return (size_t)(ncomp * mx * my * mz + mx * mx + my * my + mz * mz) * sizeof(double);
+}
+
+extern "C" int seqvp_mfma_probe(const double* a, const double* b, double* d, void* stream) {
+  if (!a || !b || !d) return SEQVP_EINVAL;
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
+  return launch_status();
+}
+
+static inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// T = compute/intermediate type. Intermediates ping-pong between the workspace and `out` (which has
+// room for ncomp*mx*my*mz doubles, so for T = float it holds them too).
+template <typename T>
+static int dst_poisson(const double* rhs, const double* den, const double* scale, int negate, int64_t mx, int64_t my,
+                       int64_t mz, int64_t ncomp, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!rhs || !den || !out || mx < 1 || my < 1 || mz < 1 || ncomp < 1 || ncomp > 65535) return SEQVP_EINVAL;
+  if (scale == out) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < seqvp_dst_workspace_bytes(mx, my, mz, ncomp)) return SEQVP_EWORKSPACE;
+  if (ncomp * mz > 65535) return SEQVP_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t total = ncomp * mx * my * mz;
+  T* tmp = (T*)ws;
+  T* Sx = tmp + total;
+  T* Sy = Sx + mx * mx;
+  T* Sz = Sy + my * my;
+  T* alt = (T*)out;
+  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(mx * mx, 256)), dim3(256), 0, s, Sx, mx);
+  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(my * my, 256)), dim3(256), 0, s, Sy, my);
+  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(mz * mz, 256)), dim3(256), 0, s, Sz, mz);
+  const int64_t O = ncomp * mz * my;
+  const double sign = negate ? -1.0 : 1.0;
+  const dim3 grow(cdiv64(O, TM), cdiv64(mx, TN));
+  // forward x (pre-scale): rhs -> tmp
+  if (scale)
+    hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_SCALE, POST_NONE>), grow, dim3(256), 0, s, rhs, scale, Sx, O,
+                       mx, nullptr, 1.0, tmp);
+  else
+    hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_NONE, POST_NONE>), grow, dim3(256), 0, s, rhs, nullptr, Sx, O,
+                       mx, nullptr, 1.0, tmp);
+  // forward y: tmp -> alt (batch ncomp*mz, inner mx)
+  const dim3 gy(cdiv64(mx, TN), cdiv64(my, TM), ncomp * mz);
+  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gy, dim3(256), 0, s, tmp, Sy, my, mx, nullptr, alt);
+  // forward z + division by the spectrum: alt -> tmp (batch ncomp, inner my*mx)
+  const dim3 gz(cdiv64(my * mx, TN), cdiv64(mz, TM), ncomp);
+  hipLaunchKernelGGL((dst_cols_kernel<T, POST_DIV>), gz, dim3(256), 0, s, alt, Sz, mz, my * mx, den, tmp);
+  // inverse z: tmp -> alt
+  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gz, dim3(256), 0, s, tmp, Sz, mz, my * mx, nullptr, alt);
+  // inverse y: alt -> tmp
+  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gy, dim3(256), 0, s, alt, Sy, my, mx, nullptr, tmp);
+  // inverse x + sign*(s*x) in binary64: tmp -> out
+  hipLaunchKernelGGL((dst_rows_kernel<T, T, double, PRE_NONE, POST_SCALE>), grow, dim3(256), 0, s, tmp, nullptr, Sx, O,
+                     mx, scale, sign, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_dst_poisson(const double* rhs, const double* den, const double* scale, int negate, int64_t mx,
+                                 int64_t my, int64_t mz, int64_t ncomp, double* out, void* ws, size_t ws_bytes,
+                                 void* stream) {
+  return dst_poisson<double>(rhs, den, scale, negate, mx, my, mz, ncomp, out, ws, ws_bytes, stream);
+}
+
+extern "C" int seqvp_dst_poisson_f32(const double* rhs, const double* den, const double* scale, int negate, int64_t mx,
+                                     int64_t my, int64_t mz, int64_t ncomp, double* out, void* ws, size_t ws_bytes,
+                                     void* stream) {
+  return dst_poisson<float>(rhs, den, scale, negate, mx, my, mz, ncomp, out, ws, ws_bytes, stream);
+}

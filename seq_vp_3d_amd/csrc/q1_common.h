@@ -1,0 +1,245 @@
+// Shared device code of the Q1 cell sweeps (gfx950, wave64).
+//
+// Layouts (same as jd_curl.three_d): node fields (nz, ny, nx, 3) f64 with components xyz and x
+// fastest; cell c = (z0*cy + y0)*cx + x0 with (cz, cy, cx) = (nz-1, ny-1, nx-1); vertex a = x + 2y + 4z
+// (jd_curl/three_d/cell_q1.py:10); point p = 9*iz + 3*iy + ix (cell_q1.py:114).
+//
+// A cell is held as its 36 edge differences e[j][r][k]: the j-direction edge of component r whose
+// two other-axis bits are k = k0 + 2*k1 (other axes in increasing order), divided by h_j. The Q1
+// derivative at reference point s is the bilinear blend of the four parallel edges
+// (scripts/gen_triton_q1.py:20-65 states the same edge-difference form).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace seqvp {
+
+// Constant vector layout shared with the host (39 doubles; triton_q1_kernels.py:1-6 layout).
+enum : int {
+  K_IH = 0, K_GAUSS = 3, K_QW = 6, K_MU = 33, K_NU = 34, K_W = 35, K_DELTA = 36, K_ACT = 37, K_SCALE = 38,
+  K_LEN = 39
+};
+
+// Gauss nodes .5 -+ sqrt(15)/10 exactly as float64 (cell_q1.py:106).
+__device__ constexpr double GAUSS_S[3] = {0x1.cda042f0236e0p-4, 0.5, 0x1.c64bf7a1fb924p-1};
+__device__ constexpr double CLOSED_S[3] = {0.0, 0.5, 1.0};
+
+struct Dims {
+  int64_t nx, ny, nz;   // nodes
+  int64_t cx, cy, cz;   // cells
+  int64_t ncell;
+};
+
+__host__ __device__ inline Dims make_dims(int64_t nx, int64_t ny, int64_t nz) {
+  Dims d;
+  d.nx = nx; d.ny = ny; d.nz = nz;
+  d.cx = nx - 1; d.cy = ny - 1; d.cz = nz - 1;
+  d.ncell = d.cx * d.cy * d.cz;
+  return d;
+}
+
+// Node index of vertex a of the cell whose vertex 0 is node b.
+__device__ __forceinline__ int64_t vertex_node(int64_t b, int a, const Dims& d) {
+  return b + (a & 1) + ((a >> 1) & 1) * d.nx + ((a >> 2) & 1) * d.nx * d.ny;
+}
+
+__device__ __forceinline__ int64_t cell_base_node(int64_t c, const Dims& d) {
+  const int64_t x0 = c % d.cx;
+  const int64_t y0 = (c / d.cx) % d.cy;
+  const int64_t z0 = c / (d.cx * d.cy);
+  return (z0 * d.ny + y0) * d.nx + x0;
+}
+
+// Other axes of direction j, increasing: j=0 -> (1,2), j=1 -> (0,2), j=2 -> (0,1).
+__host__ __device__ constexpr int other1(int j) { return j == 0 ? 1 : 0; }
+__host__ __device__ constexpr int other2(int j) { return j == 2 ? 1 : 2; }
+// Vertices (a0, a1) of the j-edge with other-axis bits k0 (axis other1) and k1 (axis other2).
+__host__ __device__ constexpr int edge_a0(int j, int k) {
+  return ((k & 1) << other1(j)) | (((k >> 1) & 1) << other2(j));
+}
+__host__ __device__ constexpr int edge_a1(int j, int k) { return edge_a0(j, k) | (1 << j); }
+
+struct Edges {
+  double e[3][3][4];   // [j][r][k]
+};
+
+// Loads the 8 vertices (24 doubles) of a cell.
+__device__ __forceinline__ void load_cell(const double* __restrict__ f, int64_t b, const Dims& d, double v[8][3]) {
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const double* p = f + 3 * vertex_node(b, a, d);
+    v[a][0] = p[0];
+    v[a][1] = p[1];
+    v[a][2] = p[2];
+  }
+}
+
+__device__ __forceinline__ void make_edges(const double v[8][3], const double ih[3], Edges& E) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        E.e[j][r][k] = (v[edge_a1(j, k)][r] - v[edge_a0(j, k)][r]) * ih[j];
+}
+
+// V[r][j] = sum_k e[j][r][k] f(k0, s_o1) f(k1, s_o2) at reference point s = (sx, sy, sz).
+__device__ __forceinline__ void point_grad(const Edges& E, const double s[3], double V[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double so1 = s[other1(j)], so2 = s[other2(j)];
+    const double w00 = (1.0 - so1) * (1.0 - so2), w10 = so1 * (1.0 - so2);
+    const double w01 = (1.0 - so1) * so2, w11 = so1 * so2;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      V[r][j] = (E.e[j][r][0] * w00 + E.e[j][r][1] * w10) + (E.e[j][r][2] * w01 + E.e[j][r][3] * w11);
+  }
+}
+
+// F = I + V; cofactor rows A_r = F_{r+1} x F_{r+2}; J = F_0 . A_0 (geometry.py:11-23, 60-89).
+__device__ __forceinline__ double jacobian(const double V[3][3]) {
+  const double f00 = V[0][0] + 1.0, f11 = V[1][1] + 1.0, f22 = V[2][2] + 1.0;
+  const double a00 = f11 * f22 - V[1][2] * V[2][1];
+  const double a01 = V[1][2] * V[2][0] - V[1][0] * f22;
+  const double a02 = V[1][0] * V[2][1] - f11 * V[2][0];
+  return (f00 * a00 + V[0][1] * a01) + V[0][2] * a02;
+}
+
+__device__ __forceinline__ double cofactor_jacobian(const double V[3][3], double A[3][3]) {
+  double F[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) F[r][j] = V[r][j] + (r == j ? 1.0 : 0.0);
+  A[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+  A[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+  A[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+  A[1][0] = F[2][1] * F[0][2] - F[2][2] * F[0][1];
+  A[1][1] = F[2][2] * F[0][0] - F[2][0] * F[0][2];
+  A[1][2] = F[2][0] * F[0][1] - F[2][1] * F[0][0];
+  A[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+  A[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+  A[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+  return (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
+}
+
+__device__ __forceinline__ void curl_of(const double V[3][3], double C[3]) {
+  C[0] = V[2][1] - V[1][2];
+  C[1] = V[0][2] - V[2][0];
+  C[2] = V[1][0] - V[0][1];
+}
+
+// ---- compensated (double-double) accumulation -------------------------------------------------
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void dd_add(DD& x, double v) {
+  double s, e;
+  two_sum(x.hi, v, s, e);
+  x.hi = s;
+  x.lo += e;
+}
+
+__device__ __forceinline__ void dd_add(DD& x, const DD& y) {
+  double s, e;
+  two_sum(x.hi, y.hi, s, e);
+  x.hi = s;
+  x.lo += e + y.lo;
+}
+
+__device__ __forceinline__ DD dd_shfl_xor(const DD& x, int mask) {
+  DD y;
+  y.hi = __shfl_xor(x.hi, mask, 64);
+  y.lo = __shfl_xor(x.lo, mask, 64);
+  return y;
+}
+
+// Wave64 butterfly reduction; every lane ends with the same total (fixed order, deterministic).
+__device__ __forceinline__ DD dd_wave_sum(DD x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    DD y = dd_shfl_xor(x, m);
+    dd_add(x, y);
+  }
+  return x;
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = fmin(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = fmax(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
+// NaN-propagating min (a NaN J must never look like a large value).
+__device__ __forceinline__ double nan_min(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b);
+}
+
+__device__ __forceinline__ double wave_nan_min(double x) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = nan_min(x, __shfl_xor(x, m, 64));
+  return x;
+}
+
+// Block ids are dealt to the 8 XCDs round robin; remap so each XCD walks one contiguous range of
+// cells (neighbouring cells share vertices through that XCD's L2).
+__device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
+  const int64_t per = (nblocks + 7) / 8;
+  const int64_t xcd = bid % 8, slot = bid / 8;
+  const int64_t full = nblocks - 8 * (per - 1);   // XCDs that hold `per` blocks
+  const int64_t start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return start + slot;
+}
+
+// ---- localized C2 barrier (barrier.py:19-36; triton_q1_kernels _barrier_*) ----------------------
+// r = max(act - J, 0)/scale, t = (min(J, act) - delta)/scale; log t through log1p(-r) when r < 1/2.
+__device__ __forceinline__ double barrier_logt(double r, double t) { return r < 0.5 ? log1p(-r) : log(t); }
+
+__device__ __forceinline__ double barrier_value(double J, double w, double delta, double act, double scale) {
+  if (!(J < act)) return 0.0;
+  const double r = (act - J) / scale, t = (J - delta) / scale;
+  return -w * r * r * barrier_logt(r, t);
+}
+
+__device__ __forceinline__ double barrier_first(double J, double w, double delta, double act, double scale) {
+  if (!(J < act)) return 0.0;
+  const double r = (act - J) / scale, t = (J - delta) / scale;
+  const double lt = barrier_logt(r, t);
+  return w * (2.0 * r * lt - r * r / t) / scale;
+}
+
+__device__ __forceinline__ double barrier_second(double J, double w, double delta, double act, double scale) {
+  if (!(J < act)) return 0.0;
+  const double r = (act - J) / scale, t = (J - delta) / scale;
+  const double lt = barrier_logt(r, t);
+  const double q = r / t;
+  return w * (-2.0 * lt + 4.0 * q + q * q) / (scale * scale);
+}
+
+// Unweighted stable psi(Jb) - psi(Ja) (energy_comparison.py:14-46, localized branch).
+__device__ __forceinline__ double barrier_change(double Ja, double Jb, double delta, double act, double scale) {
+  const double a = fmin(Ja, act), b = fmin(Jb, act);
+  if (a == act && b == act) return 0.0;
+  const double ra = (act - a) / scale, rb = (act - b) / scale;
+  const double h = (b - a) / scale;
+  const double loga = ra < 0.5 ? log1p(-ra) : log((a - delta) / scale);
+  const double x = (b - a) / (a - delta);
+  const double lr = x >= -0.5 ? log1p(x) : log((b - delta) / (a - delta));
+  return h * (ra + rb) * loga - rb * rb * lr;
+}
+
+}  // namespace seqvp

@@ -1,0 +1,910 @@
+// Q1 cell sweeps for gfx950: soft JD/curl and closed27 barrier energies, the energy gradient, the
+// GGN diagonal, the T62 node JD measure and the T66 global first JD root.
+//
+// One cell per lane, 256-lane workgroups (4 wave64), block ids remapped per XCD. FP64 VALU bound:
+// 8-27 kflop per cell against 72-216 B of node reads. Energies reduce in double-double (wave
+// butterfly, then a fixed-order block and grid pass). Node gathers are deterministic: each cell
+// writes its 24 vertex contributions to a structure-of-arrays buffer loc[a*3+r][cell] (coalesced
+// stores), and one lane per node adds the eight contributions in vertex order a = 0..7 after the
+// node base -- no atomics, so every result is bit-reproducible.
+#include "q1_common.h"
+#include "abi_util.h"
+
+namespace seqvp {
+
+constexpr int BLOCK = 256;
+
+// ------------------------------------------------------------------------------------------------
+// Block and grid reductions of NS double-double sums and NM NaN-propagating minima.
+template <int NS, int NM>
+__device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
+  __shared__ double sh[BLOCK / 64][2 * NS + NM + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NS; ++q) s[q] = dd_wave_sum(s[q]);
+#pragma unroll
+  for (int q = 0; q < NM; ++q) m[q] = wave_nan_min(m[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) { sh[wave][2 * q] = s[q].hi; sh[wave][2 * q + 1] = s[q].lo; }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) sh[wave][2 * NS + q] = m[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      DD t{sh[0][2 * q], sh[0][2 * q + 1]};
+      for (int w = 1; w < BLOCK / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
+      out[2 * q] = t.hi;
+      out[2 * q + 1] = t.lo;
+    }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+      double t = sh[0][2 * NS + q];
+      for (int w = 1; w < BLOCK / 64; ++w) t = nan_min(t, sh[w][2 * NS + q]);
+      out[2 * NS + q] = t;
+    }
+  }
+}
+
+// One workgroup folds all block partials (stride 2*NS+NM) in a fixed order; out gets NS sums
+// (hi + lo) then NM minima, or, with INTERLEAVE (NS == NM), sum0, min0, sum1, min1, ...
+template <int NS, int NM, bool INTERLEAVE = false>
+__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
+                                                                double* __restrict__ out) {
+  constexpr int STRIDE = 2 * NS + NM;
+  DD s[NS];
+  double m[NM > 0 ? NM : 1];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) s[q] = DD{0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < NM; ++q) m[q] = __builtin_inf();
+  for (int64_t b = threadIdx.x; b < nblocks; b += BLOCK) {
+    const double* p = partials + b * STRIDE;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) dd_add(s[q], DD{p[2 * q], p[2 * q + 1]});
+#pragma unroll
+    for (int q = 0; q < NM; ++q) m[q] = nan_min(m[q], p[2 * NS + q]);
+  }
+  __shared__ double fin[STRIDE];
+  block_reduce_store<NS, NM>(s, m, fin);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) out[INTERLEAVE ? 2 * q : q] = fin[2 * q] + fin[2 * q + 1];
+#pragma unroll
+    for (int q = 0; q < NM; ++q) out[INTERLEAVE ? 2 * q + 1 : NS + q] = fin[2 * NS + q];
+  }
+}
+
+__device__ __forceinline__ void load_ih(const double* __restrict__ k, double ih[3]) {
+  ih[0] = k[K_IH];
+  ih[1] = k[K_IH + 1];
+  ih[2] = k[K_IH + 2];
+}
+
+__device__ __forceinline__ void gauss_point(int p, double s[3]) {
+  s[0] = GAUSS_S[p % 3];
+  s[1] = GAUSS_S[(p / 3) % 3];
+  s[2] = GAUSS_S[p / 9];
+}
+
+__device__ __forceinline__ void closed_point(int p, double s[3]) {
+  s[0] = CLOSED_S[p % 3];
+  s[1] = CLOSED_S[(p / 3) % 3];
+  s[2] = CLOSED_S[p / 9];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Soft JD/curl change energy (residuals.py:166-172) and its stable difference against a base
+// (energy_comparison.py:196-200, blocks as in residuals.py:96-103).
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK) soft_energy_kernel(const double* __restrict__ u, const double* __restrict__ ref,
+                                                            const double* __restrict__ base,
+                                                            const double* __restrict__ k, Dims d,
+                                                            double* __restrict__ partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  if (c < d.ncell) {
+    double ih[3];
+    load_ih(k, ih);
+    const double mu = k[K_MU], nu = k[K_NU];
+    const int64_t b = cell_base_node(c, d);
+    Edges Eu, Er, Eb;
+    {
+      double v[8][3];
+      load_cell(u, b, d, v);
+      make_edges(v, ih, Eu);
+      load_cell(ref, b, d, v);
+      make_edges(v, ih, Er);
+      if (WITH_BASE) {
+        load_cell(base, b, d, v);
+        make_edges(v, ih, Eb);
+      }
+    }
+    double cj = 0.0, cc = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 27; ++p) {
+      double sp[3];
+      gauss_point(p, sp);
+      const double qw = k[K_QW + p];
+      double V[3][3], W[3][3], C[3], Cr[3];
+      point_grad(Eu, sp, V);
+      const double J = jacobian(V);
+      curl_of(V, C);
+      point_grad(Er, sp, W);
+      const double Jr = jacobian(W);
+      curl_of(W, Cr);
+      const double dj = J - Jr;
+      const double d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
+      cj += qw * dj * dj;
+      cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
+      if (WITH_BASE) {
+        double Y[3][3], Ca[3];
+        point_grad(Eb, sp, Y);
+        const double Ja = jacobian(Y);
+        curl_of(Y, Ca);
+        const double sm = sqrt(mu * qw), sn = sqrt(nu * qw);
+        const double xj = sm * (Ja - Jr), yj = sm * dj;
+        dd_add(s[2], 0.5 * ((yj - xj) * (yj + xj)));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double xc = sn * (Ca[i] - Cr[i]), yc = sn * (C[i] - Cr[i]);
+          dd_add(s[2], 0.5 * ((yc - xc) * (yc + xc)));
+        }
+      }
+    }
+    dd_add(s[0], 0.5 * mu * cj);
+    dd_add(s[1], 0.5 * nu * cc);
+  }
+  block_reduce_store<3, 0>(s, none, partials + blk * 6);
+}
+
+// Closed27 barrier value, min J, unweighted stable change vs base, min base J
+// (residuals.py:173-186; energy_comparison.py:234-240; triton_q1_kernels barrier_energy).
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __restrict__ u,
+                                                               const double* __restrict__ base,
+                                                               const double* __restrict__ k, Dims d,
+                                                               double* __restrict__ partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double m[2] = {__builtin_inf(), __builtin_inf()};
+  if (c < d.ncell) {
+    double ih[3];
+    load_ih(k, ih);
+    const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+    const int64_t b = cell_base_node(c, d);
+    Edges Eu, Eb;
+    {
+      double v[8][3];
+      load_cell(u, b, d, v);
+      make_edges(v, ih, Eu);
+      if (WITH_BASE) {
+        load_cell(base, b, d, v);
+        make_edges(v, ih, Eb);
+      }
+    }
+    double val = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < 27; ++p) {
+      double sp[3];
+      closed_point(p, sp);
+      double V[3][3];
+      point_grad(Eu, sp, V);
+      const double J = jacobian(V);
+      m[0] = nan_min(m[0], J);
+      val += barrier_value(J, w, delta, act, scale);
+      if (WITH_BASE) {
+        double Y[3][3];
+        point_grad(Eb, sp, Y);
+        const double Ja = jacobian(Y);
+        m[1] = nan_min(m[1], Ja);
+        if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+      }
+    }
+    dd_add(s[0], val);
+  }
+  block_reduce_store<2, 2>(s, m, partials + blk * 6);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-cell vertex contributions (SoA: loc[(a*3+r)*ncell + c]).
+//
+// Gauss soft VJP: flux X = A*jd + curl adjoint with jd = mu qw (J-Jr), curl = nu qw (c-cr)
+// (linearization.py:48-66,159-166); closed27 barrier VJP: X = A psi'(J) (linearization.py:169-177).
+// The transpose accumulates G[j][r][k] += w_k(s) X[r][j] over points, then
+// loc[a][r] = sum_j (+-1/h_j) G[j][r][k(a)]  (the adjoint of the edge-difference form).
+struct EdgeAdj {
+  double g[3][3][4];
+};
+
+__device__ __forceinline__ void adj_zero(EdgeAdj& G) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) G.g[j][r][q] = 0.0;
+}
+
+__device__ __forceinline__ void adj_accumulate(EdgeAdj& G, const double s[3], const double X[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double so1 = s[other1(j)], so2 = s[other2(j)];
+    const double w00 = (1.0 - so1) * (1.0 - so2), w10 = so1 * (1.0 - so2);
+    const double w01 = (1.0 - so1) * so2, w11 = so1 * so2;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      G.g[j][r][0] += w00 * X[r][j];
+      G.g[j][r][1] += w10 * X[r][j];
+      G.g[j][r][2] += w01 * X[r][j];
+      G.g[j][r][3] += w11 * X[r][j];
+    }
+  }
+}
+
+// Vertex a's k index for direction j: the bits of a on the two other axes.
+__device__ __forceinline__ constexpr int vertex_k(int a, int j) {
+  return ((a >> other1(j)) & 1) | (((a >> other2(j)) & 1) << 1);
+}
+
+__device__ __forceinline__ void adj_store(const EdgeAdj& G, const double ih[3], double* __restrict__ loc, int64_t c,
+                                          int64_t ncell) {
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double t = ih[j] * G.g[j][r][vertex_k(a, j)];
+        v = ((a >> j) & 1) ? v + t : v - t;
+      }
+      loc[(int64_t)(a * 3 + r) * ncell + c] = v;
+    }
+}
+
+__device__ __forceinline__ void soft_vjp(const Edges& Eu, const Edges& Er, const double* __restrict__ k, EdgeAdj& G) {
+  const double mu = k[K_MU], nu = k[K_NU];
+#pragma unroll 1
+  for (int p = 0; p < 27; ++p) {
+    double sp[3];
+    gauss_point(p, sp);
+    const double qw = k[K_QW + p];
+    double V[3][3], A[3][3], W[3][3], C[3], Cr[3];
+    point_grad(Eu, sp, V);
+    const double J = cofactor_jacobian(V, A);
+    curl_of(V, C);
+    point_grad(Er, sp, W);
+    const double Jr = jacobian(W);
+    curl_of(W, Cr);
+    const double jd = mu * qw * (J - Jr);
+    const double c0 = nu * qw * (C[0] - Cr[0]), c1 = nu * qw * (C[1] - Cr[1]), c2 = nu * qw * (C[2] - Cr[2]);
+    double X[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * jd;
+    X[2][1] += c0;
+    X[1][2] -= c0;
+    X[0][2] += c1;
+    X[2][0] -= c1;
+    X[1][0] += c2;
+    X[0][1] -= c2;
+    adj_accumulate(G, sp, X);
+  }
+}
+
+__device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __restrict__ k, EdgeAdj& G) {
+  const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+#pragma unroll 1
+  for (int p = 0; p < 27; ++p) {
+    double sp[3];
+    closed_point(p, sp);
+    double V[3][3], A[3][3];
+    point_grad(Eu, sp, V);
+    const double J = cofactor_jacobian(V, A);
+    const double b1 = barrier_first(J, w, delta, act, scale);
+    double X[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * b1;
+    adj_accumulate(G, sp, X);
+  }
+}
+
+// MODE 0: soft (+ barrier) into one buffer; MODE 1: barrier only; MODE 2: soft into loc, barrier into
+// loc2 (the joint GF/GE form, linearization.py:217-238).
+template <int MODE, bool SOFT, bool BARRIER>
+__global__ void __launch_bounds__(BLOCK) grad_locals_kernel(const double* __restrict__ u, const double* __restrict__ ref,
+                                                            const double* __restrict__ k, Dims d,
+                                                            double* __restrict__ loc, double* __restrict__ loc2) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (c >= d.ncell) return;
+  double ih[3];
+  load_ih(k, ih);
+  const int64_t b = cell_base_node(c, d);
+  Edges Eu;
+  {
+    double v[8][3];
+    load_cell(u, b, d, v);
+    make_edges(v, ih, Eu);
+  }
+  EdgeAdj G;
+  adj_zero(G);
+  if (SOFT) {
+    Edges Er;
+    double v[8][3];
+    load_cell(ref, b, d, v);
+    make_edges(v, ih, Er);
+    soft_vjp(Eu, Er, k, G);
+  }
+  if (MODE == 2) {
+    adj_store(G, ih, loc, c, d.ncell);
+    adj_zero(G);
+  }
+  if (BARRIER) barrier_vjp(Eu, k, G);
+  adj_store(G, ih, MODE == 2 ? loc2 : loc, c, d.ncell);
+}
+
+// GGN diagonal per cell (linearization.py:357-380; gen_triton_q1.py:447-515):
+// sum_Gauss qw (mu (A_r . dN_a)^2 + nu sum_{j!=r} dN_a,j^2) + sum_closed27 psi''(J) (A_r . dN_a)^2.
+// dN_a,j(s) = (+-1/h_j) f(a_o1, s_o1) f(a_o2, s_o2) (cell_q1.py:118-127).
+__device__ __forceinline__ void basis_grad(int a, const double s[3], const double ih[3], double dn[3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int o1 = other1(j), o2 = other2(j);
+    const double f1 = ((a >> o1) & 1) ? s[o1] : 1.0 - s[o1];
+    const double f2 = ((a >> o2) & 1) ? s[o2] : 1.0 - s[o2];
+    const double t = ih[j] * f1 * f2;
+    dn[j] = ((a >> j) & 1) ? t : -t;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restrict__ u, const double* __restrict__ k,
+                                                           Dims d, double* __restrict__ loc) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (c >= d.ncell) return;
+  double ih[3];
+  load_ih(k, ih);
+  const double mu = k[K_MU], nu = k[K_NU];
+  const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+  const int64_t b = cell_base_node(c, d);
+  Edges Eu;
+  {
+    double v[8][3];
+    load_cell(u, b, d, v);
+    make_edges(v, ih, Eu);
+  }
+  double acc[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) acc[a][r] = 0.0;
+#pragma unroll 1
+  for (int p = 0; p < 27; ++p) {
+    double sp[3];
+    gauss_point(p, sp);
+    const double qw = k[K_QW + p];
+    double V[3][3], A[3][3];
+    point_grad(Eu, sp, V);
+    cofactor_jacobian(V, A);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      double dn[3];
+      basis_grad(a, sp, ih, dn);
+      const double sq0 = dn[0] * dn[0], sq1 = dn[1] * dn[1], sq2 = dn[2] * dn[2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double l = (A[r][0] * dn[0] + A[r][1] * dn[1]) + A[r][2] * dn[2];
+        const double cs = r == 0 ? sq1 + sq2 : (r == 1 ? sq0 + sq2 : sq0 + sq1);
+        acc[a][r] += qw * (mu * (l * l) + nu * cs);
+      }
+    }
+  }
+#pragma unroll 1
+  for (int p = 0; p < 27; ++p) {
+    double sp[3];
+    closed_point(p, sp);
+    double V[3][3], A[3][3];
+    point_grad(Eu, sp, V);
+    const double J = cofactor_jacobian(V, A);
+    const double b2 = barrier_second(J, w, delta, act, scale);
+    if (b2 != 0.0) {
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        double dn[3];
+        basis_grad(a, sp, ih, dn);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const double l = (A[r][0] * dn[0] + A[r][1] * dn[1]) + A[r][2] * dn[2];
+          acc[a][r] += b2 * (l * l);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) loc[(int64_t)(a * 3 + r) * d.ncell + c] = acc[a][r];
+}
+
+// out[n,r] = base[n,r] + sum_{a=0..7} loc[a*3+r][cell(n,a)] (+ loc2 likewise), fixed vertex order
+// (the reference gathers in the same order, gen_triton_q1.py:425-444). base may be NULL.
+template <bool TWO>
+__global__ void __launch_bounds__(BLOCK) gather_kernel(const double* __restrict__ base,
+                                                       const double* __restrict__ loc,
+                                                       const double* __restrict__ loc2, Dims d,
+                                                       double* __restrict__ out) {
+  const int64_t nnode = d.nx * d.ny * d.nz;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nnode) return;
+  const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
+  double acc[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) acc[r] = base ? base[3 * i + r] : 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t xa = x - (a & 1), ya = y - ((a >> 1) & 1), za = z - ((a >> 2) & 1);
+    if (xa < 0 || xa >= d.cx || ya < 0 || ya >= d.cy || za < 0 || za >= d.cz) continue;
+    const int64_t cell = (za * d.cy + ya) * d.cx + xa;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) acc[r] += loc[(int64_t)(a * 3 + r) * d.ncell + cell];
+  }
+  if (TWO) {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int64_t xa = x - (a & 1), ya = y - ((a >> 1) & 1), za = z - ((a >> 2) & 1);
+      if (xa < 0 || xa >= d.cx || ya < 0 || ya >= d.cy || za < 0 || za >= d.cz) continue;
+      const int64_t cell = (za * d.cy + ya) * d.cx + xa;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) acc[r] += loc2[(int64_t)(a * 3 + r) * d.ncell + cell];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[3 * i + r] = acc[r];
+}
+
+// ------------------------------------------------------------------------------------------------
+// T62 cell minima: min(closed27 J, exact edge-quadratic minima) (jd_affine.py:17-48).
+__device__ __forceinline__ double edge_quadratic_min(double f0, double f1, double f2) {
+  const double a = 2.0 * (f0 - 2.0 * f1 + f2);
+  const double b = -3.0 * f0 + 4.0 * f1 - f2;
+  const bool inside = (a > 0.0) && (b < 0.0) && (-b < 2.0 * a);
+  return inside ? f0 - b * b / (4.0 * a) : __builtin_inf();
+}
+
+__global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __restrict__ u, const double* __restrict__ k,
+                                                            Dims d, double* __restrict__ minima) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (c >= d.ncell) return;
+  double ih[3];
+  load_ih(k, ih);
+  const int64_t b = cell_base_node(c, d);
+  Edges Eu;
+  {
+    double v[8][3];
+    load_cell(u, b, d, v);
+    make_edges(v, ih, Eu);
+  }
+  double J[27];
+#pragma unroll
+  for (int p = 0; p < 27; ++p) {
+    double sp[3], V[3][3];
+    closed_point(p, sp);
+    point_grad(Eu, sp, V);
+    J[p] = jacobian(V);
+  }
+  double m = J[0];
+#pragma unroll
+  for (int p = 1; p < 27; ++p) m = fmin(m, J[p]);
+  // 12 edges through the closed27 points (jd_affine.py:17-21): x-lines, y-lines, z-lines.
+#pragma unroll
+  for (int iz = 0; iz < 3; iz += 2)
+#pragma unroll
+    for (int iy = 0; iy < 3; iy += 2) m = fmin(m, edge_quadratic_min(J[9 * iz + 3 * iy], J[9 * iz + 3 * iy + 1], J[9 * iz + 3 * iy + 2]));
+#pragma unroll
+  for (int iz = 0; iz < 3; iz += 2)
+#pragma unroll
+    for (int ix = 0; ix < 3; ix += 2) m = fmin(m, edge_quadratic_min(J[9 * iz + ix], J[9 * iz + 3 + ix], J[9 * iz + 6 + ix]));
+#pragma unroll
+  for (int iy = 0; iy < 3; iy += 2)
+#pragma unroll
+    for (int ix = 0; ix < 3; ix += 2) m = fmin(m, edge_quadratic_min(J[3 * iy + ix], J[9 + 3 * iy + ix], J[18 + 3 * iy + ix]));
+  minima[c] = m;
+}
+
+__global__ void __launch_bounds__(BLOCK) node_min_kernel(const double* __restrict__ minima, Dims d,
+                                                         double* __restrict__ node) {
+  const int64_t nnode = d.nx * d.ny * d.nz;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nnode) return;
+  const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
+  double m = __builtin_inf();
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t xa = x - (a & 1), ya = y - ((a >> 1) & 1), za = z - ((a >> 2) & 1);
+    if (xa < 0 || xa >= d.cx || ya < 0 || ya >= d.cy || za < 0 || za >= d.cz) continue;
+    m = fmin(m, minima[(za * d.cy + ya) * d.cx + xa]);
+  }
+  node[i] = m;
+}
+
+// ------------------------------------------------------------------------------------------------
+// T66 first root in (0, 1] of b0 + b1 s + b2 s^2 + b3 s^3 with b0 > 0 (global_jd_step.py:22-87).
+__device__ double first_root(double b0, double b1, double b2, double b3) {
+  const double eps = 2.220446049250313e-16;
+  const double nanv = __builtin_nan("");
+  // critical points of 3 b3 s^2 + 2 b2 s + b1 in (0, 1)
+  const double a = 3.0 * b3, bb = 2.0 * b2, cc = b1;
+  const bool linear = a == 0.0;
+  const double r_lin = (linear && bb != 0.0) ? -cc / bb : nanv;
+  const double disc = bb * bb - 4.0 * a * cc;
+  const double sq = sqrt(disc >= 0.0 ? disc : 0.0);
+  const double q = -(bb + (bb >= 0.0 ? sq : -sq)) / 2.0;
+  double r1 = (!linear && disc >= 0.0 && a != 0.0) ? q / a : nanv;
+  double r2 = (!linear && disc >= 0.0 && q != 0.0) ? cc / q : nanv;
+  r1 = linear ? r_lin : r1;
+  r2 = linear ? nanv : r2;
+  r1 = (r1 > 0.0 && r1 < 1.0) ? r1 : 1.0;
+  r2 = (r2 > 0.0 && r2 < 1.0) ? r2 : 1.0;
+  const double t1 = fmin(r1, r2), t2 = fmax(r1, r2);
+  auto qv = [&](double s) { return b0 + s * (b1 + s * (b2 + s * b3)); };
+  auto touch = [&](double s) {
+    return 4.0 * eps * (fabs(b0) + s * (fabs(b1) + s * (fabs(b2) + s * fabs(b3))));
+  };
+  const bool q1 = qv(t1) <= touch(t1), q2 = qv(t2) <= touch(t2), q3 = qv(1.0) <= 0.0;
+  double hi, lo;
+  if (q1) { hi = t1; lo = 0.0; }
+  else if (q2) { hi = t2; lo = t1; }
+  else if (q3) { hi = 1.0; lo = t2; }
+  else return __builtin_inf();
+  for (int it = 0; it < 50; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (qv(mid) <= 0.0) hi = mid; else lo = mid;
+  }
+  return lo;
+}
+
+// cofactor rows of a plain 3x3 (no identity): R_r = M_{r+1} x M_{r+2} (geometry.py:11-23).
+__device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int j = (r + 1) % 3, kk = (r + 2) % 3;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int l = (cc + 1) % 3, m = (cc + 2) % 3;
+      A[r][cc] = M[j][l] * M[kk][m] - M[j][m] * M[kk][l];
+    }
+  }
+}
+
+// Per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129; torch_ops.py:636-673).
+__global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict__ u, const double* __restrict__ v,
+                                                        const double* __restrict__ k, double L, Dims d,
+                                                        double* __restrict__ partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  double best = __builtin_inf();
+  double invalid = 0.0, bad = 0.0;
+  if (c < d.ncell) {
+    double ih[3];
+    load_ih(k, ih);
+    const double delta = k[K_DELTA];
+    const double L2 = L * L, L3 = L * L * L;
+    const int64_t b = cell_base_node(c, d);
+    Edges Eu, Ev;
+    {
+      double w[8][3];
+      load_cell(u, b, d, w);
+      make_edges(w, ih, Eu);
+      load_cell(v, b, d, w);
+      make_edges(w, ih, Ev);
+    }
+    for (int p = 0; p < 27; ++p) {
+      double sp[3];
+      closed_point(p, sp);
+      double F[3][3], V[3][3], AF[3][3], AV[3][3];
+      point_grad(Eu, sp, F);
+      F[0][0] += 1.0;
+      F[1][1] += 1.0;
+      F[2][2] += 1.0;
+      point_grad(Ev, sp, V);
+      cofactor_plain(F, AF);
+      cofactor_plain(V, AV);
+      const double c0 = ((F[0][0] * AF[0][0] + F[0][1] * AF[0][1]) + F[0][2] * AF[0][2]) - delta;
+      double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          c1 += AF[r][j] * V[r][j];
+          c2 += F[r][j] * AV[r][j];
+        }
+      const double c3 = (V[0][0] * AV[0][0] + V[0][1] * AV[0][1]) + V[0][2] * AV[0][2];
+      const double b1 = c1 * L, b2 = c2 * L2, b3 = c3 * L3;
+      if (!(isfinite(c0) && isfinite(b1) && isfinite(b2) && isfinite(b3))) {
+        bad = 1.0;
+        continue;
+      }
+      if (!(c0 > 0.0)) {
+        invalid += 1.0;
+        continue;
+      }
+      if (c0 > fabs(b1) + fabs(b2) + fabs(b3)) continue;
+      best = fmin(best, first_root(c0, b1, b2, b3));
+    }
+  }
+  // min over the block; counts summed (exact integers in double)
+  __shared__ double sh[BLOCK / 64][3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  best = wave_min(best);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    invalid += __shfl_xor(invalid, m, 64);
+    bad = fmax(bad, __shfl_xor(bad, m, 64));
+  }
+  if (lane == 0) {
+    sh[wave][0] = best;
+    sh[wave][1] = invalid;
+    sh[wave][2] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b0 = sh[0][0], i0 = sh[0][1], f0 = sh[0][2];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      b0 = fmin(b0, sh[w][0]);
+      i0 += sh[w][1];
+      f0 = fmax(f0, sh[w][2]);
+    }
+    partials[blk * 3 + 0] = b0;
+    partials[blk * 3 + 1] = i0;
+    partials[blk * 3 + 2] = f0;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) jd_root_reduce_kernel(const double* __restrict__ partials, int64_t nblocks,
+                                                               double* __restrict__ out) {
+  double best = __builtin_inf(), invalid = 0.0, bad = 0.0;
+  for (int64_t b = threadIdx.x; b < nblocks; b += BLOCK) {
+    best = fmin(best, partials[3 * b]);
+    invalid += partials[3 * b + 1];
+    bad = fmax(bad, partials[3 * b + 2]);
+  }
+  __shared__ double sh[BLOCK / 64][3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  best = wave_min(best);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    invalid += __shfl_xor(invalid, m, 64);
+    bad = fmax(bad, __shfl_xor(bad, m, 64));
+  }
+  if (lane == 0) {
+    sh[wave][0] = best;
+    sh[wave][1] = invalid;
+    sh[wave][2] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b0 = sh[0][0], i0 = sh[0][1], f0 = sh[0][2];
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      b0 = fmin(b0, sh[w][0]);
+      i0 += sh[w][1];
+      f0 = fmax(f0, sh[w][2]);
+    }
+    out[0] = b0;
+    out[1] = i0;
+    out[2] = f0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Node-term energies: image, proximity and their stable differences (residuals.py:82-94,189-195).
+__global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __restrict__ warped,
+                                                            const double* __restrict__ fixed,
+                                                            const double* __restrict__ dw,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ ref,
+                                                            const double* __restrict__ bwarped,
+                                                            const double* __restrict__ bu, double lam, double omega,
+                                                            Dims d, double* __restrict__ partials) {
+  const int64_t nnode = d.nx * d.ny * d.nz;
+  DD s[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nnode; i += stride) {
+    const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
+    const double tx = (x == 0 || x == d.nx - 1) ? 0.5 : 1.0;
+    const double ty = (y == 0 || y == d.ny - 1) ? 0.5 : 1.0;
+    const double tz = (z == 0 || z == d.nz - 1) ? 0.5 : 1.0;
+    const double q = omega * tz * ty * tx;
+    const double qd = dw ? q * dw[i] : q;
+    const double r = warped[i] - fixed[i];
+    dd_add(s[0], 0.5 * (qd * r * r));
+    double pr = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double e = u[3 * i + c] - ref[3 * i + c];
+      pr += q * (e * e);
+    }
+    dd_add(s[1], 0.5 * lam * pr);
+    if (bwarped) {
+      const double sq = sqrt(qd);
+      const double xi = sq * (bwarped[i] - fixed[i]), yi = sq * r;
+      dd_add(s[2], 0.5 * ((yi - xi) * (yi + xi)));
+      const double sl = sqrt(lam * q);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double xp = sl * (bu[3 * i + c] - ref[3 * i + c]), yp = sl * (u[3 * i + c] - ref[3 * i + c]);
+        dd_add(s[3], 0.5 * ((yp - xp) * (yp + xp)));
+      }
+    }
+  }
+  block_reduce_store<4, 0>(s, none, partials + (int64_t)blockIdx.x * 8);
+}
+
+}  // namespace seqvp
+
+// ================================================================================================
+// C-ABI
+using namespace seqvp;
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nblocks = cdiv(d.ncell, BLOCK);
+  switch (op) {
+    case 0: return (size_t)nblocks * 6 * sizeof(double);
+    case 1: return (size_t)nblocks * 6 * sizeof(double);
+    case 2: return (size_t)d.ncell * 24 * sizeof(double);
+    case 3: return (size_t)d.ncell * 24 * sizeof(double);
+    case 4: return (size_t)d.ncell * 48 * sizeof(double);
+    case 5: return (size_t)nblocks * 3 * sizeof(double);
+    case 8: return (size_t)d.ncell * sizeof(double);
+    case 9: return (size_t)1024 * 8 * sizeof(double);
+    default: return 0;
+  }
+}
+
+size_t seqvp_q1_workspace_bytes(int op, int64_t nx, int64_t ny, int64_t nz) { return q1_workspace(op, nx, ny, nz); }
+
+static bool grid_ok(int64_t nx, int64_t ny, int64_t nz) { return nx >= 2 && ny >= 2 && nz >= 2; }
+
+extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base, const double* k,
+                                    int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                    void* stream) {
+  if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(0, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  if (base)
+    hipLaunchKernelGGL((soft_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
+  else
+    hipLaunchKernelGGL((soft_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_barrier_energy(const double* u, const double* base, const double* k, int64_t nx, int64_t ny,
+                                       int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(1, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  if (base)
+    hipLaunchKernelGGL((barrier_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, base, k, d, part);
+  else
+    hipLaunchKernelGGL((barrier_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, part);
+  // partials [val hi, val lo, chg hi, chg lo, minJ, min base J] -> out [val, minJ, chg, min base J]
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_gradient(const double* node_base, const double* u, const double* ref, int soft, int barrier,
+                                 const double* k, int64_t nx, int64_t ny, int64_t nz, double* out, void* ws,
+                                 size_t ws_bytes, void* stream) {
+  if (!u || !k || !out || !grid_ok(nx, ny, nz) || (soft && !ref)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(2, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* loc = (double*)ws;
+  if (!soft && !barrier) {
+    hipLaunchKernelGGL(copy_or_zero_kernel, dim3(cdiv(3 * nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, node_base, out,
+                       3 * nx * ny * nz);
+    return launch_status();
+  }
+  if (soft && barrier)
+    hipLaunchKernelGGL((grad_locals_kernel<0, true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, k, d, loc, loc);
+  else if (soft)
+    hipLaunchKernelGGL((grad_locals_kernel<0, true, false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, k, d, loc, loc);
+  else
+    hipLaunchKernelGGL((grad_locals_kernel<1, false, true>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, loc, loc);
+  hipLaunchKernelGGL((gather_kernel<false>), dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, node_base, loc,
+                     loc, d, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_joint(const double* outer_base, const double* total_base, const double* u, const double* ref,
+                              const double* k, int64_t nx, int64_t ny, int64_t nz, double* outer, double* total,
+                              void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !ref || !k || !outer || !total || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(4, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* soft = (double*)ws;
+  double* bar = soft + 24 * d.ncell;
+  hipLaunchKernelGGL((grad_locals_kernel<2, true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, k, d, soft, bar);
+  const int64_t ng = cdiv(nx * ny * nz, BLOCK);
+  hipLaunchKernelGGL((gather_kernel<false>), dim3(ng), dim3(BLOCK), 0, s, outer_base, bar, bar, d, outer);
+  hipLaunchKernelGGL((gather_kernel<true>), dim3(ng), dim3(BLOCK), 0, s, total_base, soft, bar, d, total);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_ggn_diagonal(const double* node_base, const double* u, const double* k, int64_t nx,
+                                     int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(3, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* loc = (double*)ws;
+  hipLaunchKernelGGL(ggn_locals_kernel, dim3(nb), dim3(BLOCK), 0, s, u, k, d, loc);
+  hipLaunchKernelGGL((gather_kernel<false>), dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, node_base, loc,
+                     loc, d, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_node_min_jd(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz, double* node,
+                                 void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !k || !node || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(8, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  hipStream_t s = (hipStream_t)stream;
+  double* minima = (double*)ws;
+  hipLaunchKernelGGL(cell_min_jd_kernel, dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, k, d, minima);
+  hipLaunchKernelGGL(node_min_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, minima, d, node);
+  return launch_status();
+}
+
+extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, double limit, int64_t nx, int64_t ny,
+                             int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !v || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(jd_root_kernel, dim3(nb), dim3(BLOCK), 0, s, u, v, k, limit, d, part);
+  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_node_energy(const double* warped, const double* fixed, const double* data_weight,
+                                 const double* u, const double* ref, const double* base_warped, const double* base_u,
+                                 double lam, double hx, double hy, double hz, int64_t nx, int64_t ny, int64_t nz,
+                                 double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!warped || !fixed || !u || !ref || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if ((base_warped == nullptr) != (base_u == nullptr)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(9, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nb = 1024;   // grid-stride; 1024 partials of 8 doubles
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(node_energy_kernel, dim3(nb), dim3(BLOCK), 0, s, warped, fixed, data_weight, u, ref, base_warped,
+                     base_u, lam, hx * hy * hz, d, part);
+  hipLaunchKernelGGL((reduce_partials_kernel<4, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}

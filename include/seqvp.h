@@ -63,6 +63,14 @@ int seqvp_node_energy(const double* warped, const double* fixed, const double* d
                       double hx, double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* out,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* Node terms of the gradient and the GGN diagonal (replaces the node-term lines of
+ * TorchLinearization.gradient / joint_outer_gradients / diagonal, torch_ops.py:563-568, 591-600, 784-791;
+ * linearization.py:150-156, 362): image = qd (W-F) b, total = image + lam q (u-ref), ggn = qd b^2 + lam q,
+ * with q the trapezoid node mass and qd = q*data_weight (data_weight NULL = 1). Outputs may be NULL. */
+int seqvp_node_terms(const double* warped, const double* fixed, const double* data_weight, const double* image_grad,
+                     const double* u, const double* ref, double lam, double hx, double hy, double hz, int64_t nx,
+                     int64_t ny, int64_t nz, double* image, double* total, double* ggn, void* stream);
+
 /* Energy gradient on the nodes (replaces FusedQ1.gradient_nodes, triton_q1.py:504-518;
  * linearization.py:159-201): out = node_base + [Gauss JD/curl VJP] + [closed27 barrier VJP].
  * node_base may be NULL (zero). out may alias node_base. out is (nz,ny,nx,3). */
@@ -158,6 +166,14 @@ int seqvp_restrict(const double* u, int64_t nx, int64_t ny, int64_t nz, double* 
  * (poisson_gradient.py:1403-1437). */
 int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t nx, int64_t ny, int64_t nz,
                       double* theta_out, double* u, void* stream);
+
+/* PG metric equilibration scale on the free vector (replaces the scale lines of _pg_metric_inverse,
+ * poisson_gradient.py:472-479, 507-515, and affine_factors, jd_affine.py:228-242):
+ * scale = (sqrt(m)/sqrt(diag)) * r, r = min(1, max(J_a, floor)/act) with J_a the node JD measure
+ * (node may be NULL: r = 1). diag/scale are (3, nz-2, ny-2, nx-2); bad[0] (int64, device) counts
+ * free entries with diag <= 0 or nonfinite diag / J_a (the reference raises FloatingPointError). */
+int seqvp_metric_scale(const double* diag, const double* node, double m, double act, double floor_, int64_t nx,
+                       int64_t ny, int64_t nz, double* scale, int64_t* bad, void* stream);
 
 /* One v_mfma_f64_16x16x4f64 on A (16x4, row-major a) and B (4x16, row-major b) written to d (16x16,
  * row-major) through the lane mapping the DST kernels assume; a layout self-check. */

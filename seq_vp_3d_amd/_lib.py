@@ -20,6 +20,8 @@ SIGNATURES = {
     "seqvp_q1_barrier_energy": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_node_energy": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64,
                           c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_node_terms": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp,
+                         c_vp, c_vp, c_vp],
     "seqvp_q1_gradient": [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_joint": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_ggn_diagonal": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
@@ -37,6 +39,7 @@ SIGNATURES = {
     "seqvp_inject": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_metric_scale": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_mfma_probe": [c_vp, c_vp, c_vp, c_vp],
 }
 _RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz}
@@ -65,8 +68,35 @@ def load():
     return _LIB
 
 
+class CallTimer:
+    """Records a pair of HIP events (on the current stream) around every C-ABI call while active."""
+
+    def __init__(self):
+        self.events = {}
+
+    def summary(self):
+        """name -> (calls, mean ms) over the recorded calls (synchronizes on the last event)."""
+        out = {}
+        for name, pairs in self.events.items():
+            if pairs:
+                pairs[-1][1].synchronize()
+                ms = [a.elapsed_time(b) for a, b in pairs]
+                out[name] = (len(ms), sum(ms) / len(ms))
+        return out
+
+
+TIMER = None
+
+
 def call(name, *args):
+    timer = TIMER
+    if timer is not None:
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
     code = getattr(load(), name)(*args)
+    if timer is not None:
+        end.record()
+        timer.events.setdefault(name, []).append((start, end))
     if code != 0:
         raise SeqvpError(f"{name}: {ERRORS.get(code, code)}")
 

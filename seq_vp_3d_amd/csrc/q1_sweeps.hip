@@ -749,6 +749,38 @@ __global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __rest
   block_reduce_store<4, 0>(s, none, partials + (int64_t)blockIdx.x * 8);
 }
 
+// Node terms of the gradient and of the GGN diagonal (torch_ops.py:563-568, 784-791;
+// linearization.py:150-156, 362): image = qd (W - F) b; total = image + lam q (u - ref);
+// ggn = qd b^2 + lam q. Any output may be NULL.
+__global__ void __launch_bounds__(BLOCK) node_terms_kernel(const double* __restrict__ warped,
+                                                           const double* __restrict__ fixed,
+                                                           const double* __restrict__ dw,
+                                                           const double* __restrict__ bgrad,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ ref, double lam, double omega,
+                                                           Dims d, double* __restrict__ image,
+                                                           double* __restrict__ total, double* __restrict__ ggn) {
+  const int64_t nnode = d.nx * d.ny * d.nz;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nnode) return;
+  const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
+  const double tx = (x == 0 || x == d.nx - 1) ? 0.5 : 1.0;
+  const double ty = (y == 0 || y == d.ny - 1) ? 0.5 : 1.0;
+  const double tz = (z == 0 || z == d.nz - 1) ? 0.5 : 1.0;
+  const double q = omega * tz * ty * tx;
+  const double qd = dw ? q * dw[i] : q;
+  const double lq = lam * q;
+  const double r = qd * (warped[i] - fixed[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double b = bgrad[3 * i + c];
+    const double im = r * b;
+    if (image) image[3 * i + c] = im;
+    if (total) total[3 * i + c] = im + lq * (u[3 * i + c] - ref[3 * i + c]);
+    if (ggn) ggn[3 * i + c] = qd * (b * b) + lq;
+  }
+}
+
 }  // namespace seqvp
 
 // ================================================================================================
@@ -906,5 +938,17 @@ extern "C" int seqvp_node_energy(const double* warped, const double* fixed, cons
   hipLaunchKernelGGL(node_energy_kernel, dim3(nb), dim3(BLOCK), 0, s, warped, fixed, data_weight, u, ref, base_warped,
                      base_u, lam, hx * hy * hz, d, part);
   hipLaunchKernelGGL((reduce_partials_kernel<4, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_node_terms(const double* warped, const double* fixed, const double* data_weight,
+                                const double* image_grad, const double* u, const double* ref, double lam, double hx,
+                                double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* image, double* total,
+                                double* ggn, void* stream) {
+  if (!warped || !fixed || !image_grad || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (total && (!u || !ref)) return SEQVP_EINVAL;
+  const Dims d = make_dims(nx, ny, nz);
+  hipLaunchKernelGGL(node_terms_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, warped,
+                     fixed, data_weight, image_grad, u, ref, lam, hx * hy * hz, d, image, total, ggn);
   return launch_status();
 }

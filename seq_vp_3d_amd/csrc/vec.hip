@@ -109,6 +109,44 @@ __global__ void __launch_bounds__(VB) axpy_inject_kernel(const double* __restric
   }
 }
 
+// PG metric equilibration scale on the free vector (poisson_gradient.py:472-479, 507-515;
+// jd_affine.py:228-242): s = sqrt(m)/sqrt(d) * r with r = min(1, max(J_a, floor)/act) from the node JD
+// measure (node == NULL: r = 1). bad[0] counts entries with d <= 0, nonfinite d or nonfinite J_a.
+__global__ void __launch_bounds__(VB) metric_scale_kernel(const double* __restrict__ diag,
+                                                          const double* __restrict__ node, double sqrt_m, double act,
+                                                          double floor_, int64_t nx, int64_t ny, int64_t nz,
+                                                          double* __restrict__ scale,
+                                                          unsigned long long* __restrict__ bad) {
+  const int64_t mx = nx - 2, my = ny - 2, mz = nz - 2, F = mx * my * mz;
+  const int64_t f = (int64_t)blockIdx.x * VB + threadIdx.x;
+  int nbad = 0;
+  if (f < F) {
+    double r = 1.0;
+    if (node) {
+      const int64_t x = f % mx + 1, y = (f / mx) % my + 1, z = f / (mx * my) + 1;
+      const double ja = node[(z * ny + y) * nx + x];
+      if (!isfinite(ja)) nbad = 1;
+      r = fmin(1.0, fmax(ja, floor_) / act);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double d = diag[c * F + f];
+      if (!(d > 0.0) || !isfinite(d)) nbad = 1;
+      scale[c * F + f] = (sqrt_m / sqrt(d)) * r;
+    }
+  }
+  __shared__ int sh[VB / 64];
+  int t = nbad;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) t += __shfl_xor(t, m, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = sh[0] + sh[1] + sh[2] + sh[3];
+    if (tot) atomicAdd(bad, (unsigned long long)tot);
+  }
+}
+
 }  // namespace seqvp
 
 using namespace seqvp;
@@ -160,5 +198,16 @@ extern "C" int seqvp_axpy_inject(const double* theta, const double* p, double ta
   const int64_t nn = nx * ny * nz;
   hipLaunchKernelGGL(axpy_inject_kernel, dim3((nn + VB - 1) / VB), dim3(VB), 0, (hipStream_t)stream, theta, p, tau, nx,
                      ny, nz, theta_out, u);
+  return launch_status();
+}
+
+extern "C" int seqvp_metric_scale(const double* diag, const double* node, double m, double act, double floor_,
+                                  int64_t nx, int64_t ny, int64_t nz, double* scale, int64_t* bad, void* stream) {
+  if (!diag || !scale || !bad || nx < 3 || ny < 3 || nz < 3 || !(m > 0.0) || (node && !(act > 0.0))) return SEQVP_EINVAL;
+  const int64_t F = (nx - 2) * (ny - 2) * (nz - 2);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(copy_or_zero_kernel, dim3(1), dim3(256), 0, s, nullptr, (double*)bad, 1);
+  hipLaunchKernelGGL(metric_scale_kernel, dim3((F + VB - 1) / VB), dim3(VB), 0, s, diag, node, sqrt(m), act, floor_, nx,
+                     ny, nz, scale, (unsigned long long*)bad);
   return launch_status();
 }

@@ -1,0 +1,229 @@
+"""Device-resident state of one registration pair and the hot-path operations on it.
+
+Every array of the pair lives in HBM for the whole solve (fields, images, spline coefficients,
+metric, direction, BB history); each operation below is one or a few libseqvp_hip.so calls, and
+only scalars (energies, roots, dot products, counts) come back to the host -- the reference moves
+160-165 MB per gradient, direction and promotion across PCIe (accelerated.py:1525, 2801-2874;
+hybrid.py:262-286).
+
+Operations and the reference code they follow:
+  metric()     -- _pg_metric_inverse with the image_shift_ggn_equilibrated_jd_affine policy
+                  (poisson_gradient.py:400-576) without the T69 face/interior extension
+  gradient()   -- TorchLinearization.gradient (torch_ops.py:559-589) restricted to theta
+  direction()  -- p = -S M^-1 S g (poisson_gradient.py:1241-1262, hybrid.py:262-286)
+  jd_bound()   -- T66 global root on the direction (torch_ops.py:636-673)
+  trial()      -- evaluate_trial's device part: candidate map, prescreen, sampling, energies vs base
+                  (accelerated.py:1666-1730, 1574-1664)
+  promote()    -- promote_trial: the gradient sampling of an accepted trial (accelerated.py:2850-2874)
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .poisson import DevicePoisson, build_poisson_inverse
+from .prescreen import prescreen_flags
+from .problem import q1_coefficients
+from .q1 import FusedJDRoot, FusedQ1
+from .spline import DeviceSpline, build_image_sampler
+
+JD_AFFINE_FLOOR = 1e-12
+JD_AFFINE_ACTIVATION = .25
+
+
+@dataclass
+class Energies:
+    image: float
+    proximity: float
+    jd: float
+    curl: float
+    barrier: float
+
+    @property
+    def total(self):
+        return self.image + self.proximity + self.jd + self.curl + self.barrier
+
+    @property
+    def outer(self):
+        return self.image + self.barrier
+
+
+@dataclass
+class TrialResult:
+    ok: bool
+    reason: str
+    energy: Energies | None
+    total_difference: float = float("nan")     # E(trial) - E(base), stable
+    outer_difference: float = float("nan")     # F(trial) - F(base), stable
+    min_gap: float = float("nan")
+    unproven_cells: int = 0
+
+
+class PairState:
+    """Device buffers of one pair on one grid (node fields (nz, ny, nx, 3), free vectors (3F,))."""
+
+    def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None):
+        self.problem = problem
+        self.grid = problem.grid
+        self.device = torch.device(device)
+        self.q1 = FusedQ1(problem, self.device)
+        self.root = FusedJDRoot(problem, q1=self.q1)
+        g = self.grid
+        self.shape = tuple(g.shape)
+        self.nfree = 3 * g.n_free_scalar
+        t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous()
+        self.fixed = t(fixed)
+        self.data_weight = None if data_weight is None else t(data_weight)
+        mov = moving.detach().cpu().numpy() if isinstance(moving, torch.Tensor) else np.asarray(moving)
+        self.spline = DeviceSpline(sampler if sampler is not None else build_image_sampler(mov, g), g, self.device)
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)
+        self.ref = z(*self.shape, 3)                 # u_ref (proximal centre)
+        self.u = z(*self.shape, 3)                   # accepted state
+        self.theta = z(self.nfree)
+        self.warped = z(*self.shape)
+        self.bgrad = z(*self.shape, 3)
+        self.u_t = z(*self.shape, 3)                 # trial
+        self.theta_t = z(self.nfree)
+        self.warped_t = z(*self.shape)
+        self.node_a = z(*self.shape, 3)
+        self.node_b = z(*self.shape, 3)
+        self.g = z(self.nfree)
+        self.p = z(self.nfree)
+        self.v = z(*self.shape, 3)
+        self.scale = z(self.nfree)
+        self.diag = z(self.nfree)
+        self.flags = torch.empty(g.n_cells, dtype=torch.uint8, device=self.device)
+        self.count = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.bad = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.dot_out = z(4)
+        self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
+        D, _ = q1_coefficients(g.spacing, "closed27_uniform")
+        self.D = D
+        self.poisson = None
+        self.energy = None          # Energies of the accepted state
+        self.metric_info = {}
+
+    # ---- small helpers -------------------------------------------------------------------------
+    def _stream(self):
+        return _lib.stream_handle(self.device)
+
+    def dots(self, *pairs):
+        """Compensated dot products of (x, y) free-vector pairs (at most 4), as host floats."""
+        xs = (ctypes.c_void_p * len(pairs))(*(a.data_ptr() for a, _ in pairs))
+        ys = (ctypes.c_void_p * len(pairs))(*(b.data_ptr() for _, b in pairs))
+        n = pairs[0][0].numel()
+        _lib.call("seqvp_dots", xs, ys, len(pairs), n, _lib.ptr(self.dot_out), _lib.ptr(self.dot_ws),
+                  self.dot_ws.numel(), self._stream())
+        return self.dot_out[:len(pairs)].tolist()
+
+    def inject(self, theta, out):
+        nz, ny, nx = self.shape
+        _lib.call("seqvp_inject", _lib.ptr(theta), nx, ny, nz, _lib.ptr(out), self._stream())
+        return out
+
+    def restrict(self, field, out):
+        nz, ny, nx = self.shape
+        _lib.call("seqvp_restrict", _lib.ptr(field), nx, ny, nz, _lib.ptr(out), self._stream())
+        return out
+
+    def set_state(self, theta=None, ref=None):
+        """Set the accepted map from theta (and optionally the reference) and evaluate it."""
+        if ref is not None:
+            self.ref.copy_(torch.as_tensor(ref, dtype=torch.float64))
+        if theta is not None:
+            self.theta.copy_(torch.as_tensor(theta, dtype=torch.float64))
+        self.inject(self.theta, self.u)
+        _, _, bad = self.spline.warp(self.u, value=self.warped, gradient=self.bgrad)
+        if int(bad.item()):
+            raise FloatingPointError("state outside the sampling support")
+        self.energy = self.evaluate_energy(self.u, self.warped)
+
+    def evaluate_energy(self, u, warped):
+        q = self.q1
+        node = q.node_energy(warped, self.fixed, u, self.ref, self.problem.weights.lam, data_weight=self.data_weight)
+        soft = q.soft_energy(u, self.ref)
+        bar = q.barrier_energy(u)
+        n, s, b = (x.tolist() for x in (node, soft, bar))
+        if not b[1] > q.delta:
+            raise FloatingPointError("device barrier points contradict a feasible state")
+        return Energies(n[0], n[1], s[0], s[1], b[0])
+
+    # ---- metric --------------------------------------------------------------------------------
+    def metric(self, *, shift_factor=1., jd_affine=True, activation=JD_AFFINE_ACTIVATION):
+        """P^-1 = S M^-1 S with M the shifted Q1 Poisson metric (poisson_gradient.py:269-293, 400-576)."""
+        q, w = self.q1, self.problem.weights
+        q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
+                     ggn=self.node_a)
+        diag_nodes = q.diagonal_nodes(self.node_a, self.u)
+        self.restrict(diag_nodes, self.diag)
+        # alpha = factor * mean over free nodes of (q/omega) |b|^2 / 3 (q/omega = 1 on free nodes)
+        self.restrict(self.bgrad, self.g)
+        (bb,) = self.dots((self.g, self.g))
+        alpha = shift_factor * (bb / (self.nfree // 3)) / 3
+        a = w.lam + alpha
+        bp = w.mu / 2 + w.nu / 2
+        inverse = build_poisson_inverse(self.grid, a=a, b=bp, metric="q1")
+        self.poisson = DevicePoisson(inverse, self.device, ws=q.ws)
+        inv_h2 = sum(1 / h ** 2 for h in self.grid.spacing)
+        m = self.grid.omega * (a + bp * (8 / 9) * inv_h2)
+        node = q.node_min_jd(self.u) if jd_affine else None
+        nz, ny, nx = self.shape
+        _lib.call("seqvp_metric_scale", _lib.ptr(self.diag), _lib.ptr(node), float(m), float(activation),
+                  JD_AFFINE_FLOOR, nx, ny, nz, _lib.ptr(self.scale), _lib.ptr(self.bad), self._stream())
+        if int(self.bad.item()):
+            raise FloatingPointError("GGN diagonal or node JD invalid")
+        self.metric_info = dict(shift_alpha=alpha, a=a, b=bp, mass_diagonal=m)
+        return self.metric_info
+
+    # ---- gradient / direction / bound ------------------------------------------------------------
+    def gradient(self):
+        """Full energy gradient on theta (device), g = restrict(node terms + soft + barrier VJPs)."""
+        q, w = self.q1, self.problem.weights
+        q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
+                     total=self.node_a)
+        q.gradient_nodes(self.node_a, self.u, self.ref, out=self.node_b)
+        return self.restrict(self.node_b, self.g)
+
+    def direction(self, fp32=False):
+        """p = -S M^-1 (S g)."""
+        return self.poisson.apply_equilibrated(self.scale, self.g, out=self.p, fp32=fp32)
+
+    def jd_bound(self, limit):
+        self.inject(self.p, self.v)
+        return self.root(self.u, self.v, limit)
+
+    # ---- trial and promotion -----------------------------------------------------------------------
+    def trial(self, tau):
+        """Candidate theta + tau p: device prescreen, sampling and energies against the accepted state."""
+        nz, ny, nx = self.shape
+        q, w = self.q1, self.problem.weights
+        _lib.call("seqvp_axpy_inject", _lib.ptr(self.theta), _lib.ptr(self.p), float(tau), nx, ny, nz,
+                  _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
+        prescreen_flags(self.u_t, self.grid, self.D, q.delta, self.flags, self.count)
+        _, _, bad = self.spline.warp(self.u_t, with_gradient=False, value=self.warped_t)
+        node = q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
+                             base_warped=self.warped, base_u=self.u)
+        soft = q.soft_energy(self.u_t, self.ref, self.u)
+        bar = q.barrier_energy(self.u_t, self.u)
+        vals = torch.cat((node, soft, bar, bad.to(torch.float64), self.count.to(torch.float64))).tolist()
+        n, s, b, nbad, unproven = vals[0:4], vals[4:7], vals[7:11], vals[11], int(vals[12])
+        if nbad:
+            return TrialResult(False, "INVALID_DOMAIN", None, unproven_cells=unproven)
+        if not b[1] > q.delta or not np.isfinite(b[1]):
+            return TrialResult(False, "JACOBIAN_VIOLATION", None, unproven_cells=unproven)
+        energy = Energies(n[0], n[1], s[0], s[1], b[0])
+        change = q.barrier_weight * b[2]
+        total = n[2] + n[3] + s[2] + change
+        outer = n[2] + change
+        gap = min(b[1], b[3]) - q.delta
+        return TrialResult(True, "", energy, total, outer, gap, unproven)
+
+    def promote(self, energy):
+        """Make the trial the accepted state (swap buffers) and sample the image gradient there."""
+        self.u, self.u_t = self.u_t, self.u
+        self.theta, self.theta_t = self.theta_t, self.theta
+        self.warped, self.warped_t = self.warped_t, self.warped
+        self.spline.warp(self.u, value=self.warped, gradient=self.bgrad)
+        self.energy = energy

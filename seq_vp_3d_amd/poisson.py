@@ -1,0 +1,90 @@
+"""Device Q1 Poisson metric inverse via DST-I (the TorchPoisson / _HostPoisson interface).
+
+poisson_spectrum / build_poisson_inverse follow jd_curl/three_d/poisson.py:16-30, 118-131 (the
+denominator is built once per metric on the host, in float64, exactly as the reference does and
+uploaded); apply / apply_fp32 / apply_equilibrated follow torch_ops.py:98-136 and
+hybrid.py:230-286, with every transform running in libseqvp_hip.so (seqvp_dst_poisson[_f32]).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import InputValidationError
+
+OP_DST = 6
+
+
+def poisson_spectrum(grid, metric="q1"):
+    """Q1 (or seven-point) spectrum on the interior DST-I modes (poisson.py:16-30)."""
+    if metric not in ("seven_point", "q1"):
+        raise InputValidationError("metric: expected seven_point or q1")
+    ell, mass = [], []
+    for n, h in zip(grid.shape[::-1], grid.spacing):
+        theta = np.pi * np.arange(1, n - 1) / (n - 1)
+        ell.append((2 * np.sin(theta / 2) / h) ** 2)
+        mass.append((2 + np.cos(theta)) / 3)
+    ex, ey, ez = ell[0][None, None, :], ell[1][None, :, None], ell[2][:, None, None]
+    mx, my, mz = mass[0][None, None, :], mass[1][None, :, None], mass[2][:, None, None]
+    result = ex + ey + ez if metric == "seven_point" else ex * my * mz + mx * ey * mz + mx * my * ez
+    if np.any(result <= 0) or not np.all(np.isfinite(result)):
+        raise FloatingPointError("nonpositive Poisson spectrum")
+    return result
+
+
+@dataclass(frozen=True)
+class PoissonInverse3D:
+    grid: object
+    a: float
+    b: float
+    metric: str
+    denominator: np.ndarray
+
+
+def build_poisson_inverse(grid, *, a, b, metric="q1"):
+    """omega (a + b spectrum) (poisson.py:118-131)."""
+    if not (np.isfinite(a) and a >= 0 and np.isfinite(b) and b > 0):
+        raise InputValidationError("a >= 0 and b > 0 required")
+    den = grid.omega * (a + b * poisson_spectrum(grid, metric))
+    if not np.all(np.isfinite(den)) or np.any(den <= 0):
+        raise InputValidationError("Poisson coefficients/grid not representable in float64")
+    return PoissonInverse3D(grid, float(a), float(b), metric, den)
+
+
+class DevicePoisson:
+    """M^-1 on the device for one metric; same methods as TorchPoisson (torch_ops.py:107-136)."""
+
+    def __init__(self, inverse, device="cuda", ws=None):
+        self.grid = inverse.grid
+        self.device = torch.device(device)
+        self.inverse = inverse
+        self._denominator = torch.tensor(np.ascontiguousarray(inverse.denominator), dtype=torch.float64,
+                                         device=self.device)
+        self.mz, self.my, self.mx = self.grid.interior_shape
+        self.n = 3 * self.mz * self.my * self.mx
+        self.ws = ws if ws is not None else _lib.Workspace(self.device)
+
+    def _run(self, name, rhs, scale, negate, out):
+        if rhs.dtype != torch.float64 or rhs.numel() != self.n or not rhs.is_contiguous():
+            raise InputValidationError("rhs must be a contiguous float64 (3F,) device vector")
+        if scale is not None and (scale.numel() != self.n or scale.dtype != torch.float64):
+            raise InputValidationError("scale must be a float64 (3F,) device vector")
+        out = torch.empty_like(rhs) if out is None else out
+        buf, nbytes = self.ws.get(_lib.workspace_bytes(OP_DST, self.grid.shape))
+        _lib.call(name, _lib.ptr(rhs), _lib.ptr(self._denominator), _lib.ptr(scale), int(negate), self.mx, self.my,
+                  self.mz, 3, _lib.ptr(out), _lib.ptr(buf), nbytes, _lib.stream_handle(self.device))
+        return out
+
+    def apply(self, rhs, out=None):
+        """M^-1 rhs, float64 (torch_ops.py:107-115)."""
+        return self._run("seqvp_dst_poisson", rhs.reshape(-1), None, False, out)
+
+    def apply_fp32(self, rhs, out=None):
+        """The same solve with the transforms and the division in float32 (torch_ops.py:117-136)."""
+        return self._run("seqvp_dst_poisson_f32", rhs.reshape(-1), None, False, out)
+
+    def apply_equilibrated(self, scale, gradient, out=None, fp32=False):
+        """-(s * M^-1 (s * g)): the equilibrated PG direction (hybrid.py:230-286) in one call."""
+        name = "seqvp_dst_poisson_f32" if fp32 else "seqvp_dst_poisson"
+        return self._run(name, gradient.reshape(-1), scale.reshape(-1), True, out)

@@ -1,0 +1,149 @@
+"""FusedQ1: the device Q1 cell sweeps behind the reference's fused-kernel interface.
+
+Same methods, argument meaning and return layout as jd_curl/three_d/backends/triton_q1.py:244-748
+(FusedQ1): soft_energy, barrier_energy, gradient_nodes, joint, diagonal_nodes; plus node_energy
+(the image/proximity part of AcceleratedHybridBackend._energy, accelerated.py:1583-1600) and
+node_min_jd (the T62 measure, torch_ops.py:687-773). All arrays are contiguous float64 device
+tensors; node fields are (nz, ny, nx, 3). Every call goes to libseqvp_hip.so; there is no
+host fallback.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import InputValidationError, problem_constants
+
+OP_SOFT, OP_BARRIER, OP_GRAD, OP_GGN, OP_JOINT, OP_ROOT, OP_DST, OP_DOTS, OP_NODEMIN, OP_NODE = range(10)
+
+
+class FusedQ1:
+    """Constants and launches for one problem (grid, weights, barrier) on one device."""
+
+    def __init__(self, problem, device="cuda"):
+        grid, k, D, delta = problem_constants(problem)
+        self.grid = grid
+        self.shape = tuple(grid.shape)
+        self.nz, self.ny, self.nx = self.shape
+        self.device = torch.device(device)
+        self.k = torch.tensor(k, dtype=torch.float64, device=self.device)
+        self.D = torch.tensor(D, dtype=torch.float64, device=self.device)
+        self.delta = delta
+        self.barrier_weight = float(k[35])
+        self.ws = _lib.Workspace(self.device)
+        self.ncell = grid.n_cells
+
+    # ---- helpers -----------------------------------------------------------------------------
+    def _check(self, *arrays):
+        for a in arrays:
+            if (not isinstance(a, torch.Tensor) or a.dtype != torch.float64 or not a.is_contiguous()
+                    or tuple(a.shape) != self.shape + (3,) or a.device != self.device):
+                raise InputValidationError("fused kernels need contiguous float64 device node arrays")
+
+    def _dims(self):
+        return self.nx, self.ny, self.nz
+
+    def _workspace(self, op):
+        buf, n = self.ws.get(_lib.workspace_bytes(op, self.shape))
+        return _lib.ptr(buf), n
+
+    # ---- energies ----------------------------------------------------------------------------
+    def soft_energy(self, u, ref, base=None):
+        """[JD penalty, curl penalty, stable soft difference vs base] (triton_q1.py:572-581)."""
+        self._check(u, ref) if base is None else self._check(u, ref, base)
+        out = torch.empty(3, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_SOFT)
+        _lib.call("seqvp_q1_soft_energy", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
+                  *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def barrier_energy(self, u, base=None):
+        """[barrier value, min J, unweighted barrier change, min base J] (triton_q1.py:680-691)."""
+        self._check(u) if base is None else self._check(u, base)
+        out = torch.empty(4, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_BARRIER)
+        _lib.call("seqvp_q1_barrier_energy", _lib.ptr(u), _lib.ptr(base), _lib.ptr(self.k), *self._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def node_energy(self, warped, fixed, u, ref, lam, data_weight=None, base_warped=None, base_u=None):
+        """[image, proximity, image difference vs base, proximity difference vs base]."""
+        self._check(u, ref)
+        for a in (warped, fixed):
+            if a.dtype != torch.float64 or tuple(a.shape) != self.shape or not a.is_contiguous():
+                raise InputValidationError("node scalar fields must be contiguous float64 (nz, ny, nx)")
+        out = torch.empty(4, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_NODE)
+        hx, hy, hz = self.grid.spacing
+        _lib.call("seqvp_node_energy", _lib.ptr(warped), _lib.ptr(fixed), _lib.ptr(data_weight), _lib.ptr(u),
+                  _lib.ptr(ref), _lib.ptr(base_warped), _lib.ptr(base_u), float(lam), hx, hy, hz, *self._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def node_terms(self, warped, fixed, image_grad, u, ref, lam, data_weight=None, image=None, total=None, ggn=None):
+        """Node terms: image = qd (W-F) b, total = image + lam q (u - ref), ggn = qd b^2 + lam q (fills the
+        given outputs)."""
+        hx, hy, hz = self.grid.spacing
+        _lib.call("seqvp_node_terms", _lib.ptr(warped), _lib.ptr(fixed), _lib.ptr(data_weight), _lib.ptr(image_grad),
+                  _lib.ptr(u), _lib.ptr(ref), float(lam), hx, hy, hz, *self._dims(), _lib.ptr(image), _lib.ptr(total),
+                  _lib.ptr(ggn), _lib.stream_handle(self.device))
+
+    # ---- derivatives -------------------------------------------------------------------------
+    def gradient_nodes(self, base, u, ref, *, soft=True, barrier=True, out=None):
+        """base + Gauss soft VJP + closed27 barrier VJP on the nodes (triton_q1.py:504-518)."""
+        self._check(u, base) if not soft else self._check(u, ref, base)
+        out = torch.empty_like(u) if out is None else out
+        w, n = self._workspace(OP_GRAD)
+        _lib.call("seqvp_q1_gradient", _lib.ptr(base), _lib.ptr(u), _lib.ptr(ref if soft else None), int(soft),
+                  int(barrier), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def joint(self, image, total_base, u, ref):
+        """(outer = image + barrier, total = total_base + soft + barrier) sharing the cell work
+        (triton_q1.py:563-570)."""
+        self._check(image, total_base, u, ref)
+        outer, total = torch.empty_like(u), torch.empty_like(u)
+        w, n = self._workspace(OP_JOINT)
+        _lib.call("seqvp_q1_joint", _lib.ptr(image), _lib.ptr(total_base), _lib.ptr(u), _lib.ptr(ref),
+                  _lib.ptr(self.k), *self._dims(), _lib.ptr(outer), _lib.ptr(total), w, n,
+                  _lib.stream_handle(self.device))
+        return outer, total
+
+    def diagonal_nodes(self, base, u):
+        """base + exact GGN diagonal of the geometry and barrier blocks (triton_q1.py:442-458)."""
+        self._check(base, u)
+        out = torch.empty_like(u)
+        w, n = self._workspace(OP_GGN)
+        _lib.call("seqvp_q1_ggn_diagonal", _lib.ptr(base), _lib.ptr(u), _lib.ptr(self.k), *self._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def node_min_jd(self, u):
+        """T62 node JD measure (closed27 + exact edge minima), full grid shape, +inf off-grid."""
+        self._check(u)
+        out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_NODEMIN)
+        _lib.call("seqvp_node_min_jd", _lib.ptr(u), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
+                  _lib.stream_handle(self.device))
+        return out
+
+
+class FusedJDRoot:
+    """T66 global closed27 first JD root (triton_jd_root.py:26-52): (root in s-units or inf,
+    number of points with c0 <= 0, number of points)."""
+
+    def __init__(self, problem, device="cuda", q1=None):
+        self.q1 = q1 if q1 is not None else FusedQ1(problem, device)
+
+    def __call__(self, u, v, limit):
+        q1 = self.q1
+        q1._check(u, v)
+        if not (np.isfinite(limit) and limit > 0):
+            raise InputValidationError("limit must be positive and finite")
+        out = torch.empty(3, dtype=torch.float64, device=q1.device)
+        w, n = q1._workspace(OP_ROOT)
+        _lib.call("seqvp_jd_root", _lib.ptr(u), _lib.ptr(v), _lib.ptr(q1.k), float(limit), *q1._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(q1.device))
+        root, invalid, bad = out.tolist()
+        if bad:
+            raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
+        return float(root), int(invalid), 27 * q1.ncell

@@ -1,0 +1,297 @@
+"""Parity of every device entry point with the CPU oracle on small grids (runs through the C ABI)."""
+import numpy as np
+import pytest
+
+from conftest import counterexample_cell, rough_field, smooth_field
+from oracle import jd_root as ojr, poisson as opo, prescreen as ops, q1 as Q, spline as osp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+SHAPE = (9, 10, 11)
+SPACING = (1., 1.3, .8)
+
+
+def case(activation=.05, shape=SHAPE, spacing=SPACING):
+    from seq_vp_3d_amd.problem import Q1Problem, Weights3D, make_grid_3d
+    nz, ny, nx = shape
+    grid = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=spacing[0], hy=spacing[1], hz=spacing[2])
+    w = Weights3D(lam=2e-3, mu=.03, nu=.05, kappa=.002)
+    prob = Q1Problem(grid, w, 0., activation)
+    og = Q.Grid(shape, spacing)
+    model = Q.Model(og, lam=w.lam, mu=w.mu, nu=w.nu, kappa=w.kappa, delta=0., activation=activation)
+    return prob, og, model
+
+
+def fields(shape, seed):
+    ref = smooth_field(shape, .15, seed)
+    base = ref + smooth_field(shape, .3, seed + 1) + rough_field(shape, .03, seed + 2)
+    u = base + smooth_field(shape, .1, seed + 3) + rough_field(shape, .02, seed + 4)
+    return ref, base, u
+
+
+def dev(a, device):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("activation", [.05, .9])
+def test_soft_and_barrier_energy(gpu_device, activation):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case soft/barrier energy", activation, flush=True)
+    prob, og, m = case(activation)
+    ref, base, u = fields(og.shape, 20)
+    q = FusedQ1(prob, gpu_device)
+    U, R, B = dev(u, gpu_device), dev(ref, gpu_device), dev(base, gpu_device)
+    got = q.soft_energy(U, R, B).cpu().numpy()
+    jd, curl, diff = m.soft_energy(u, ref, base)
+    assert abs(got[0] - jd) <= 1e-12 * jd and abs(got[1] - curl) <= 1e-12 * curl
+    assert abs(got[2] - diff) <= 1e-12 * (jd + curl)
+    got0 = q.soft_energy(U, R).cpu().numpy()
+    assert got0[2] == 0. and abs(got0[0] - jd) <= 1e-12 * jd
+    bar = q.barrier_energy(U, B).cpu().numpy()
+    val, jmin, chg, amin = m.barrier_energy(u, base)
+    if activation == .9:
+        assert val > 0 and chg != 0
+    assert abs(bar[0] - val) <= 1e-12 * max(val, 1e-300) + 1e-300
+    assert bar[1] == jmin and bar[3] == amin
+    w = m.barrier_weight
+    assert abs(w * bar[2] - chg) <= 1e-12 * max(abs(val), abs(chg), 1e-300)
+    zero = q.soft_energy(R, R, R).cpu().numpy()
+    assert np.array_equal(zero, np.zeros(3))
+
+
+def test_energy_nan_propagates_to_min_j(gpu_device):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case()
+    _, _, u = fields(og.shape, 30)
+    u[4, 5, 5, 0] = np.nan
+    q = FusedQ1(prob, gpu_device)
+    out = q.barrier_energy(dev(u, gpu_device)).cpu().numpy()
+    assert np.isnan(out[1])
+
+
+@pytest.mark.parametrize("activation", [.05, .9])
+def test_gradient_and_joint(gpu_device, activation):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case gradient/joint", activation, flush=True)
+    prob, og, m = case(activation)
+    ref, _, u = fields(og.shape, 40)
+    nb = np.random.default_rng(41).normal(size=u.shape)
+    q = FusedQ1(prob, gpu_device)
+    U, R, NB = dev(u, gpu_device), dev(ref, gpu_device), dev(nb, gpu_device)
+    for soft, barrier in ((True, True), (True, False), (False, True)):
+        got = q.gradient_nodes(NB, U, R, soft=soft, barrier=barrier).cpu().numpy()
+        want = m.gradient_nodes(nb, u, ref, soft=soft, barrier=barrier)
+        assert rel(got - nb, want - nb) <= 1e-12, (soft, barrier)
+    img = np.random.default_rng(42).normal(size=u.shape)
+    outer, total = q.joint(dev(img, gpu_device), NB, U, R)
+    wo = m.gradient_nodes(img, u, ref, soft=False, barrier=True)
+    wt = m.gradient_nodes(nb, u, ref, soft=True, barrier=True)
+    assert rel(outer.cpu().numpy() - img, wo - img) <= 1e-12 or activation == .05 and np.allclose(outer.cpu().numpy(), wo, rtol=0, atol=1e-15)
+    assert rel(total.cpu().numpy() - nb, wt - nb) <= 1e-12
+
+
+@pytest.mark.parametrize("activation", [.05, .9])
+def test_ggn_diagonal(gpu_device, activation):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case ggn", activation, flush=True)
+    prob, og, m = case(activation)
+    _, _, u = fields(og.shape, 50)
+    b = np.random.default_rng(51).normal(size=u.shape)
+    base = m.ggn_diagonal_base(b)
+    q = FusedQ1(prob, gpu_device)
+    got = q.diagonal_nodes(dev(base, gpu_device), dev(u, gpu_device)).cpu().numpy()
+    want = m.ggn_diagonal_nodes(base, u)
+    assert rel(got - base, want - base) <= 1e-12
+
+
+def test_node_min_jd(gpu_device):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case()
+    _, _, u = fields(og.shape, 60)
+    q = FusedQ1(prob, gpu_device)
+    got = q.node_min_jd(dev(u, gpu_device)).cpu().numpy()
+    want = m.node_min_jd(u)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+
+
+def test_jd_root(gpu_device):
+    from seq_vp_3d_amd.q1 import FusedJDRoot, FusedQ1
+    print("case jd root", flush=True)
+    prob, og, m = case()
+    _, base, _ = fields(og.shape, 70)
+    v = smooth_field(og.shape, 1.5, 71) + rough_field(og.shape, .3, 72)
+    q = FusedQ1(prob, gpu_device)
+    root = FusedJDRoot(prob, q1=q)
+    for limit in (.5, 2., 10.):
+        got = root(dev(base, gpu_device), dev(v, gpu_device), limit)
+        want = ojr.global_root(base, v, og, m.closed[3], 0., limit)
+        assert got[1:] == want[1:]
+        if np.isinf(want[0]):
+            assert np.isinf(got[0])
+        else:
+            assert abs(got[0] - want[0]) <= 1e-9 * want[0], (limit, got, want)
+
+
+def test_prescreen_bitwise(gpu_device):
+    from seq_vp_3d_amd.prescreen import prescreen_flags, prescreen_lower_bounds
+    import oracle.build
+    oracle.build.build()
+    print("case prescreen", flush=True)
+    prob, og, m = case()
+    D = m.closed[3]
+    for seed, amp in ((80, .2), (81, .45), (82, .6)):
+        u = rough_field(og.shape, amp, seed) + smooth_field(og.shape, .3, seed + 100)
+        flat = u.reshape(-1, 3)
+        idx = Q.cell_indices(og)
+        flat[idx[17]] = counterexample_cell(-.05) * np.array(SPACING) + flat[idx[17][0]]
+        u[[0, -1]] = 0.
+        u[:, [0, -1]] = 0.
+        u[:, :, [0, -1]] = 0.
+        U = dev(u, gpu_device)
+        got = prescreen_lower_bounds(U, og, D).cpu().numpy()
+        want = ops.lower_bounds_c(u, og, D)
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)) or np.array_equal(got, want, equal_nan=True)
+        cells = np.array([0, 5, 17, og.n_cells - 1], dtype=np.int64)
+        got_l = prescreen_lower_bounds(U, og, D, cells).cpu().numpy()
+        assert np.array_equal(got_l, want[cells], equal_nan=True)
+        flags, count = prescreen_flags(U, og, D, 0.)
+        proven = want > 0
+        assert np.array_equal(flags.cpu().numpy().astype(bool), proven)
+        assert int(count.item()) == int((~proven).sum())
+        flags, count = prescreen_flags(U, og, D, .01)
+        with np.errstate(invalid="ignore"):
+            proven = np.nextafter(want - .01, -np.inf) > 0
+        assert np.array_equal(flags.cpu().numpy().astype(bool), proven)
+
+
+def test_spline_warp_and_points(gpu_device):
+    from seq_vp_3d_amd.spline import DeviceSpline, build_image_sampler
+    from seq_vp_3d_amd.problem import make_grid_3d
+    print("case spline", flush=True)
+    shape, spacing = (12, 13, 14), (1., 1.5, .7)
+    grid = make_grid_3d(nx=14, ny=13, nz=12, hx=1., hy=1.5, hz=.7, x0=2., y0=-1., z0=.5)
+    og = Q.Grid(shape, spacing, (2., -1., .5))
+    img = np.random.default_rng(90).normal(size=shape)
+    sp = DeviceSpline(build_image_sampler(img, grid), grid, gpu_device)
+    ref = osp.build(img, og)
+    u = smooth_field(shape, .4, 91)
+    val, grad, bad = sp.warp(dev(u, gpu_device))
+    assert int(bad.item()) == 0
+    wv, wg = osp.sample(ref, og.identity() + u)
+    assert rel(val.cpu().numpy(), wv) <= 1e-12 and rel(grad.cpu().numpy(), wg) <= 1e-12
+    # out-of-support free node is counted
+    u2 = u.copy()
+    u2[5, 5, 1, 0] = -5.
+    _, _, bad = sp.warp(dev(u2, gpu_device))
+    assert int(bad.item()) == 1
+    pts = og.identity()[1:-1, 1:-1, 1:-1].reshape(-1, 3) + .3
+    v, g = sp.sample(dev(pts, gpu_device))
+    wv, wg = osp.sample(ref, pts)
+    assert rel(v.cpu().numpy(), wv) <= 1e-12 and rel(g.cpu().numpy(), wg) <= 1e-12
+
+
+def test_mfma_f64_layout_probe(gpu_device):
+    from seq_vp_3d_amd import _lib
+    rng = np.random.default_rng(95)
+    a, b = rng.normal(size=(16, 4)), rng.normal(size=(4, 16))
+    d = torch.zeros(256, dtype=torch.float64, device=gpu_device)
+    _lib.call("seqvp_mfma_probe", _lib.ptr(dev(a, gpu_device)), _lib.ptr(dev(b, gpu_device)), _lib.ptr(d),
+              _lib.stream_handle(gpu_device))
+    np.testing.assert_allclose(d.cpu().numpy().reshape(16, 16), a @ b, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("shape", [(9, 10, 11), (20, 23, 37)])
+def test_dst_poisson(gpu_device, shape):
+    from seq_vp_3d_amd.poisson import DevicePoisson, build_poisson_inverse
+    from seq_vp_3d_amd.problem import make_grid_3d
+    print("case dst", shape, flush=True)
+    nz, ny, nx = shape
+    grid = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=1., hy=1.3, hz=.8)
+    og = Q.Grid(shape, (1., 1.3, .8))
+    inv = build_poisson_inverse(grid, a=.3, b=.04)
+    den = opo.denominator(og, .3, .04)
+    assert np.array_equal(inv.denominator, den)
+    P = DevicePoisson(inv, gpu_device)
+    rng = np.random.default_rng(96)
+    g = rng.normal(size=3 * og.n_free_scalar)
+    s = rng.uniform(.5, 2., size=g.size)
+    got = P.apply(dev(g, gpu_device)).cpu().numpy()
+    assert rel(got, opo.apply(g, den)) <= 1e-12
+    got = P.apply_equilibrated(dev(s, gpu_device), dev(g, gpu_device)).cpu().numpy()
+    assert rel(got, -(s * opo.apply(s * g, den))) <= 1e-12
+    got32 = P.apply_equilibrated(dev(s, gpu_device), dev(g, gpu_device), fp32=True).cpu().numpy()
+    assert rel(got32, -(s * opo.apply(s * g, den))) <= 2e-5
+
+
+def test_vector_ops(gpu_device):
+    from seq_vp_3d_amd import _lib
+    shape = (9, 10, 11)
+    nz, ny, nx = shape
+    og = Q.Grid(shape)
+    rng = np.random.default_rng(97)
+    theta = rng.normal(size=3 * og.n_free_scalar)
+    p = rng.normal(size=theta.size)
+    T, Pd = dev(theta, gpu_device), dev(p, gpu_device)
+    u = torch.empty(shape + (3,), dtype=torch.float64, device=gpu_device)
+    _lib.call("seqvp_inject", _lib.ptr(T), nx, ny, nz, _lib.ptr(u), _lib.stream_handle(gpu_device))
+    assert np.array_equal(u.cpu().numpy(), Q.inject_free(theta, og))
+    back = torch.empty_like(T)
+    _lib.call("seqvp_restrict", _lib.ptr(u), nx, ny, nz, _lib.ptr(back), _lib.stream_handle(gpu_device))
+    assert np.array_equal(back.cpu().numpy(), theta)
+    tout = torch.empty_like(T)
+    _lib.call("seqvp_axpy_inject", _lib.ptr(T), _lib.ptr(Pd), .37, nx, ny, nz, _lib.ptr(tout), _lib.ptr(u),
+              _lib.stream_handle(gpu_device))
+    assert np.array_equal(tout.cpu().numpy(), theta + .37 * p)
+    assert np.array_equal(u.cpu().numpy(), Q.inject_free(theta + .37 * p, og))
+    import ctypes
+    xs = (ctypes.c_void_p * 2)(T.data_ptr(), Pd.data_ptr())
+    ys = (ctypes.c_void_p * 2)(Pd.data_ptr(), Pd.data_ptr())
+    out = torch.empty(2, dtype=torch.float64, device=gpu_device)
+    ws = torch.empty(_lib.workspace_bytes(7, shape), dtype=torch.uint8, device=gpu_device)
+    _lib.call("seqvp_dots", xs, ys, 2, theta.size, _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+              _lib.stream_handle(gpu_device))
+    want = [float(np.sum(theta.astype(np.longdouble) * p)), float(np.sum(p.astype(np.longdouble) ** 2))]
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-15)
+
+
+def test_node_energy(gpu_device):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case()
+    ref, base, u = fields(og.shape, 110)
+    rng = np.random.default_rng(111)
+    fixed, warped, bw = rng.uniform(size=og.shape), rng.uniform(size=og.shape), rng.uniform(size=og.shape)
+    m.fixed = fixed
+    q = FusedQ1(prob, gpu_device)
+    out = q.node_energy(dev(warped, gpu_device), dev(fixed, gpu_device), dev(u, gpu_device), dev(ref, gpu_device),
+                        m.lam, base_warped=dev(bw, gpu_device), base_u=dev(base, gpu_device)).cpu().numpy()
+    image, prox = m.node_energies(u, ref, warped)
+    ib, pb = m.node_energies(base, ref, bw)
+    np.testing.assert_allclose(out[:2], [image, prox], rtol=1e-13)
+    np.testing.assert_allclose(out[2], image - ib, rtol=1e-10)
+    np.testing.assert_allclose(out[3], prox - pb, rtol=1e-10)
+
+
+def test_node_terms(gpu_device):
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case()
+    ref, _, u = fields(og.shape, 120)
+    rng = np.random.default_rng(121)
+    fixed, warped = rng.uniform(size=og.shape), rng.uniform(size=og.shape)
+    b = rng.normal(size=og.shape + (3,))
+    dw = rng.uniform(.5, 1., size=og.shape)
+    m.fixed = fixed
+    m.qd = m.q * dw
+    q = FusedQ1(prob, gpu_device)
+    img, tot, ggn = (torch.empty(og.shape + (3,), dtype=torch.float64, device=gpu_device) for _ in range(3))
+    q.node_terms(dev(warped, gpu_device), dev(fixed, gpu_device), dev(b, gpu_device), dev(u, gpu_device),
+                 dev(ref, gpu_device), m.lam, data_weight=dev(dw, gpu_device), image=img, total=tot, ggn=ggn)
+    np.testing.assert_allclose(tot.cpu().numpy(), m.node_gradient_base(u, ref, warped, b), rtol=1e-14, atol=1e-300)
+    np.testing.assert_allclose(img.cpu().numpy(), m.node_gradient_base(u, ref, warped, b, total=False), rtol=1e-14)
+    np.testing.assert_allclose(ggn.cpu().numpy(), m.ggn_diagonal_base(b), rtol=1e-14)

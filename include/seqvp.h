@@ -175,9 +175,10 @@ int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t 
 int seqvp_metric_scale(const double* diag, const double* node, double m, double act, double floor_, int64_t nx,
                        int64_t ny, int64_t nz, double* scale, int64_t* bad, void* stream);
 
-/* One v_mfma_f64_16x16x4f64 on A (16x4, row-major a) and B (4x16, row-major b) written to d (16x16,
- * row-major) through the lane mapping the DST kernels assume; a layout self-check. */
-int seqvp_mfma_probe(const double* a, const double* b, double* d, void* stream);
+/* One v_mfma_f64_16x16x4f64 (fp32 = 0) or v_mfma_f32_16x16x4f32 (fp32 = 1) on A (16x4, row-major a)
+ * and B (4x16, row-major b), written to d (16x16, row-major) through the lane mapping the DST kernels
+ * assume; a layout self-check. */
+int seqvp_mfma_probe(const double* a, const double* b, int fp32, double* d, void* stream);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,7 @@ SIGNATURES = {
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_metric_scale": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
-    "seqvp_mfma_probe": [c_vp, c_vp, c_vp, c_vp],
+    "seqvp_mfma_probe": [c_vp, c_vp, c_int, c_vp, c_vp],
 }
 _RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz}
 

@@ -34,6 +34,13 @@ template <> struct Acc<float> { typedef f4 type; };
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+// Operands (both types): lane l holds A[l%16][l/16] and B[l/16][l%16]. Accumulator row of register
+// r in lane group lg = l/16 (column l%16): interleaved lg + 4r for the f64 instruction, blocked
+// 4lg + r for the f32 one (checked on the device by seqvp_mfma_probe).
+template <typename T> __device__ __forceinline__ int acc_row(int lg, int r);
+template <> __device__ __forceinline__ int acc_row<double>(int lg, int r) { return lg + 4 * r; }
+template <> __device__ __forceinline__ int acc_row<float>(int lg, int r) { return 4 * lg + r; }
+
 // sine matrix of length n into S (n*n), row-major, exact angle reduction m = (j+1)(k+1) mod 2(n+1),
 // evaluated in binary64 and rounded once to T
 template <typename T>
@@ -107,14 +114,14 @@ __global__ void __launch_bounds__(256) dst_rows_kernel(const TIN* __restrict__ X
     }
     __syncthreads();
   }
-  // D layout: lane holds column li, rows 4*lk + r
+  // accumulator layout: lane holds column li, row acc_row<T>(lk, r)
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm + 16 * a + 4 * lk + r, gn = n0 + wn + 16 * b + li;
+        const int64_t gm = m0 + wm + 16 * a + acc_row<T>(lk, r), gn = n0 + wn + 16 * b + li;
         if (gm < O && gn < n) {
           const int64_t idx = gm * n + gn;
           if (POST == POST_SCALE) {
@@ -181,7 +188,7 @@ __global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, 
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm + 16 * a + 4 * lk + r, gi = n0 + wn + 16 * b + li;
+        const int64_t gm = m0 + wm + 16 * a + acc_row<T>(lk, r), gi = n0 + wn + 16 * b + li;
         if (gm < n && gi < I) {
           const int64_t idx = gm * I + gi;
           T v = acc[a][b][r];
@@ -191,18 +198,20 @@ __global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, 
       }
 }
 
-// Layout probe: D = A(16x4) * B(4x16) with one MFMA, A[i][k] = a[i*4+k], B[k][j] = b[k*16+j];
-// writes D[i][j] to d[i*16+j] using the documented lane mapping. Used by the tests to confirm the
-// operand/accumulator layout the kernels above assume.
+// Layout probe: D = A(16x4) * B(4x16) with one MFMA of type T, A[i][k] = a[i*4+k],
+// B[k][j] = b[k*16+j]; D[i][j] is written to d[i*16+j] through the lane mapping the kernels above
+// assume (operands and acc_row). The tests compare d with A*B for both instruction types.
+template <typename T>
 __global__ void mfma_probe_kernel(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ d) {
+  typedef typename Acc<T>::type AT;
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const double af = a[li * 4 + lk];
-  const double bf = b[lk * 16 + li];
-  d4 acc = d4{0.0, 0.0, 0.0, 0.0};
-  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf, acc, 0, 0, 0);
+  const T af = (T)a[li * 4 + lk];
+  const T bf = (T)b[lk * 16 + li];
+  AT acc = AT{0, 0, 0, 0};
+  acc = mfma(af, bf, acc);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) d[(4 * lk + r) * 16 + li] = acc[r];
+  for (int r = 0; r < 4; ++r) d[acc_row<T>(lk, r) * 16 + li] = (double)acc[r];
 }
 
 }  // namespace seqvp
@@ -213,9 +222,12 @@ size_t seqvp_dst_workspace_bytes(int64_t mx, int64_t my, int64_t mz, int64_t nco
   return (size_t)(ncomp * mx * my * mz + mx * mx + my * my + mz * mz) * sizeof(double);
 }
 
-extern "C" int seqvp_mfma_probe(const double* a, const double* b, double* d, void* stream) {
+extern "C" int seqvp_mfma_probe(const double* a, const double* b, int fp32, double* d, void* stream) {
   if (!a || !b || !d) return SEQVP_EINVAL;
-  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
+  if (fp32)
+    hipLaunchKernelGGL((mfma_probe_kernel<float>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
+  else
+    hipLaunchKernelGGL((mfma_probe_kernel<double>), dim3(1), dim3(64), 0, (hipStream_t)stream, a, b, d);
   return launch_status();
 }
 

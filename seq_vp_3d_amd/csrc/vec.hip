@@ -91,6 +91,7 @@ __global__ void __launch_bounds__(VB) restrict_kernel(const double* __restrict__
 __global__ void __launch_bounds__(VB) axpy_inject_kernel(const double* __restrict__ theta, const double* __restrict__ p,
                                                          double tau, int64_t nx, int64_t ny, int64_t nz,
                                                          double* __restrict__ theta_out, double* __restrict__ u) {
+#pragma clang fp contract(off)
   const int64_t nn = nx * ny * nz;
   const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
   if (i >= nn) return;
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(VB) axpy_inject_kernel(const double* __restric
   for (int c = 0; c < 3; ++c) {
     double v = 0.0;
     if (inner) {
-      v = __dadd_rn(theta[c * F + f], __dmul_rn(tau, p[c * F + f]));   // unfused, as the host axpy
+      v = theta[c * F + f] + tau * p[c * F + f];   // product and sum rounded separately (no contraction)
       if (theta_out) theta_out[c * F + f] = v;
     }
     u[3 * i + c] = v;

@@ -36,8 +36,9 @@ def prescreen_lower_bounds(u, grid, coefficients, cells=None):
                                  dtype=torch.int64, device=u.device).contiguous()
         ncells = cell_t.numel()
     out = torch.full((ncells,), float("nan"), dtype=torch.float64, device=u.device)
+    uc = u.contiguous()     # held until the launch is enqueued behind every allocation it reads
     if usable and ncells:
-        _lib.call("seqvp_prescreen", _lib.ptr(u.contiguous()), _lib.ptr(D), _lib.ptr(cell_t), ncells, nx, ny, nz,
+        _lib.call("seqvp_prescreen", _lib.ptr(uc), _lib.ptr(D), _lib.ptr(cell_t), ncells, nx, ny, nz,
                   _lib.ptr(out), _lib.stream_handle(u.device))
     return out
 
@@ -52,6 +53,7 @@ def prescreen_flags(u, grid, coefficients, delta, flags=None, count=None):
         flags.zero_()
         count.fill_(grid.n_cells)
         return flags, count
-    _lib.call("seqvp_prescreen_flags", _lib.ptr(u.contiguous()), _lib.ptr(D), float(delta), nx, ny, nz,
+    uc = u.contiguous()
+    _lib.call("seqvp_prescreen_flags", _lib.ptr(uc), _lib.ptr(D), float(delta), nx, ny, nz,
               _lib.ptr(flags), _lib.ptr(count), _lib.stream_handle(u.device))
     return flags, count

@@ -59,7 +59,8 @@ def test_soft_and_barrier_energy(gpu_device, activation):
     if activation == .9:
         assert val > 0 and chg != 0
     assert abs(bar[0] - val) <= 1e-12 * max(val, 1e-300) + 1e-300
-    assert bar[1] == jmin and bar[3] == amin
+    # minima of J: the device forms J with fused multiply-adds, so they agree to a few ulps
+    assert abs(bar[1] - jmin) <= 1e-13 * abs(jmin) and abs(bar[3] - amin) <= 1e-13 * abs(amin)
     w = m.barrier_weight
     assert abs(w * bar[2] - chg) <= 1e-12 * max(abs(val), abs(chg), 1e-300)
     zero = q.soft_energy(R, R, R).cpu().numpy()
@@ -197,14 +198,17 @@ def test_spline_warp_and_points(gpu_device):
     assert rel(v.cpu().numpy(), wv) <= 1e-12 and rel(g.cpu().numpy(), wg) <= 1e-12
 
 
-def test_mfma_f64_layout_probe(gpu_device):
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_mfma_layout_probe(gpu_device, fp32):
     from seq_vp_3d_amd import _lib
     rng = np.random.default_rng(95)
     a, b = rng.normal(size=(16, 4)), rng.normal(size=(4, 16))
     d = torch.zeros(256, dtype=torch.float64, device=gpu_device)
-    _lib.call("seqvp_mfma_probe", _lib.ptr(dev(a, gpu_device)), _lib.ptr(dev(b, gpu_device)), _lib.ptr(d),
-              _lib.stream_handle(gpu_device))
-    np.testing.assert_allclose(d.cpu().numpy().reshape(16, 16), a @ b, rtol=1e-14, atol=1e-14)
+    A, B = dev(a, gpu_device), dev(b, gpu_device)        # keep both alive until the launch has run
+    _lib.call("seqvp_mfma_probe", _lib.ptr(A), _lib.ptr(B), fp32, _lib.ptr(d), _lib.stream_handle(gpu_device))
+    torch.cuda.synchronize()
+    tol = 1e-5 if fp32 else 1e-14
+    np.testing.assert_allclose(d.cpu().numpy().reshape(16, 16), a @ b, rtol=tol, atol=tol)
 
 
 @pytest.mark.parametrize("shape", [(9, 10, 11), (20, 23, 37)])
@@ -292,6 +296,7 @@ def test_node_terms(gpu_device):
     img, tot, ggn = (torch.empty(og.shape + (3,), dtype=torch.float64, device=gpu_device) for _ in range(3))
     q.node_terms(dev(warped, gpu_device), dev(fixed, gpu_device), dev(b, gpu_device), dev(u, gpu_device),
                  dev(ref, gpu_device), m.lam, data_weight=dev(dw, gpu_device), image=img, total=tot, ggn=ggn)
-    np.testing.assert_allclose(tot.cpu().numpy(), m.node_gradient_base(u, ref, warped, b), rtol=1e-14, atol=1e-300)
+    want = m.node_gradient_base(u, ref, warped, b)
+    np.testing.assert_allclose(tot.cpu().numpy(), want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
     np.testing.assert_allclose(img.cpu().numpy(), m.node_gradient_base(u, ref, warped, b, total=False), rtol=1e-14)
     np.testing.assert_allclose(ggn.cpu().numpy(), m.ggn_diagonal_base(b), rtol=1e-14)

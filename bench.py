@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--inner", type=int, default=5)
     ap.add_argument("--shape", type=str, default="160,192,224")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("driver", "fixed"), default="driver")
     return ap.parse_args()
 
 
@@ -134,16 +135,32 @@ def main():
     setup_s = time.perf_counter() - t0
     print(f"setup {setup_s:.1f}s amplitude {pair.amplitude:.3f} E0 {st.energy.total:.6e}", file=sys.stderr, flush=True)
     stats = dict(trials=0, accepted=0)
-    for w in range(args.warmup):
-        outer_step(st, args.inner, stats)
-        print(f"warmup {w} done E {st.energy.total:.6e}", file=sys.stderr, flush=True)
+    if args.workload == "fixed":
+        def run(n, label):
+            for k in range(n):
+                outer_step(st, args.inner, stats)
+                print(f"{label} {k} done E {st.energy.total:.6e}", file=sys.stderr, flush=True)
+    else:
+        from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+        inner_cfg = PGConfig(max_iterations=args.inner)
+
+        def run(n, label):
+            def obs(row):
+                stats["trials"] += row["inner_trials"]
+                stats["accepted"] += row["inner_accepted_steps"]
+                print(f"{label} outer {row['outer_i']} {row['reason']} inner {row['inner_status']} "
+                      f"steps {row['inner_accepted_steps']} trials {row['inner_trials']} E {row['E_candidate']} "
+                      f"GF {row['GF_candidate']} {row['seconds']*1e3:.1f} ms", file=sys.stderr, flush=True)
+            res = solve_sequential(st, SequentialConfig(), inner_cfg, observer=obs, max_outer=n)
+            stats["status"] = res.status
+            if res.outer_accepted < n:
+                raise SystemExit(f"outer loop stopped early: {res.status} {res.reason}")
+    run(args.warmup, "warmup")
     torch.cuda.synchronize()
     _lib.TIMER = _lib.CallTimer()
     stats = dict(trials=0, accepted=0)
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        outer_step(st, args.inner, stats)
-        print(f"step {s} done E {st.energy.total:.6e}", file=sys.stderr, flush=True)
+    run(args.steps, "step")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     timer, _lib.TIMER = _lib.TIMER, None
@@ -160,8 +177,12 @@ def main():
         metric="ms/outer-step @160x192x224", value=round(ms_per_step, 3), unit="ms/outer-step", n_gpus=1,
         steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=False,
         scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
-        config=dict(workload="outer step of the plain FP64 hot path (metric build + "
-                             f"{args.inner} PG iterations, trial + prescreen + promotion each)",
+        config=dict(workload=("outer step of the certified FP64 PG solver (inner solve of up to "
+                              f"{args.inner} accepted PG steps with BB2, T66 root bound, Armijo/resolution "
+                              "backtracking, prescreen-certified trials; outer E/F comparison, commit, reference "
+                              "refresh)") if args.workload == "driver" else
+                             ("fixed outer-step sequence of the FP64 hot path (metric build + "
+                              f"{args.inner} PG iterations, trial + prescreen + promotion each)"),
                     shape_zyx=list(shape), pair="synthetic native recipe seed 1", trials=stats["trials"],
                     accepted=stats["accepted"]),
         roofline=dict(bound="mfma", achieved=round(achieved, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",

@@ -167,6 +167,25 @@ int seqvp_restrict(const double* u, int64_t nx, int64_t ny, int64_t nz, double* 
 int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t nx, int64_t ny, int64_t nz,
                       double* theta_out, double* u, void* stream);
 
+/* BB2 secant products (replaces the host products of _step_proposal, poisson_gradient.py:955-990):
+ * with s = theta - theta_old, y = g - g_old, iy = p_old - p (each rounded once), out[0] = s.y,
+ * out[1] = s.s, out[2] = y.y, out[3] = y.iy, compensated (double-double) sums. */
+int seqvp_bb2_products(const double* theta, const double* theta_old, const double* g, const double* g_old,
+                       const double* p_old, const double* p, int64_t n, double* out, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* Exact linear node-domain step bound (replaces sampling_step_bound, poisson_gradient.py:993-1011):
+ * out[0] = min over free nodes/components of (hi - x)/p (p > 0) and (lo - x)/p (p < 0), x = X + u;
+ * +inf when p = 0. p is the free vector, u the node field, origin/spacing the grid's. */
+int seqvp_step_bound(const double* p, const double* u, double ox, double oy, double oz, double hx, double hy,
+                     double hz, int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* out[0] = max|a - b|, out[1] = max|b| over n entries (NaN propagates) -- the relative step of
+ * step_ratio (solvers/common.py:63-73). */
+int seqvp_max_abs_diff(const double* a, const double* b, int64_t n, double* out, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
 /* PG metric equilibration scale on the free vector (replaces the scale lines of _pg_metric_inverse,
  * poisson_gradient.py:472-479, 507-515, and affine_factors, jd_affine.py:228-242):
  * scale = (sqrt(m)/sqrt(diag)) * r, r = min(1, max(J_a, floor)/act) with J_a the node JD measure

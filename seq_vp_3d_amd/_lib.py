@@ -39,6 +39,10 @@ SIGNATURES = {
     "seqvp_inject": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_bb2_products": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_step_bound": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz,
+                         c_vp],
+    "seqvp_max_abs_diff": [c_vp, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_metric_scale": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_mfma_probe": [c_vp, c_vp, c_int, c_vp, c_vp],
 }

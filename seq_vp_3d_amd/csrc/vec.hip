@@ -148,6 +148,126 @@ __global__ void __launch_bounds__(VB) metric_scale_kernel(const double* __restri
   }
 }
 
+// BB2 secant products (poisson_gradient.py:955-990): s = theta - theta_old, y = g - g_old,
+// iy = p_old - p (each rounded once, as the host subtractions), then compensated s.y, s.s, y.y, y.iy.
+__global__ void __launch_bounds__(VB) bb2_kernel(const double* __restrict__ th, const double* __restrict__ th0,
+                                                 const double* __restrict__ g, const double* __restrict__ g0,
+                                                 const double* __restrict__ p0, const double* __restrict__ p, int64_t n,
+                                                 double* __restrict__ partials) {
+  DD acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += stride) {
+    const double sv = th[i] - th0[i], yv = g[i] - g0[i], iy = p0[i] - p[i];
+    const double pr[4][2] = {{sv, yv}, {sv, sv}, {yv, yv}, {yv, iy}};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double prod = pr[q][0] * pr[q][1];
+      const double err = fma(pr[q][0], pr[q][1], -prod);
+      dd_add(acc[q], prod);
+      acc[q].lo += err;
+    }
+  }
+  __shared__ double sh[VB / 64][8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = dd_wave_sum(acc[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sh[wave][2 * q] = acc[q].hi;
+      sh[wave][2 * q + 1] = acc[q].lo;
+    }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      DD t{sh[0][2 * q], sh[0][2 * q + 1]};
+      for (int w = 1; w < VB / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
+      partials[(int64_t)blockIdx.x * 8 + 2 * q] = t.hi;
+      partials[(int64_t)blockIdx.x * 8 + 2 * q + 1] = t.lo;
+    }
+}
+
+// Exact linear node-domain step bound (poisson_gradient.py:993-1011): min over free nodes and components of
+// (hi - x)/p for p > 0 and (lo - x)/p for p < 0 with x = X + u; +inf if none. Also max |p| (for
+// diagnostics). min/max are order independent, so the result equals the host scan's.
+__global__ void __launch_bounds__(VB) step_bound_kernel(const double* __restrict__ p, const double* __restrict__ u,
+                                                        double ox, double oy, double oz, double hx, double hy,
+                                                        double hz, int64_t nx, int64_t ny, int64_t nz,
+                                                        double* __restrict__ partials) {
+  const int64_t mx = nx - 2, my = ny - 2, mz = nz - 2, F = mx * my * mz;
+  const double lo[3] = {ox, oy, oz};
+  const double hi[3] = {ox + hx * (double)(nx - 1), oy + hy * (double)(ny - 1), oz + hz * (double)(nz - 1)};
+  const double org[3] = {ox, oy, oz}, h[3] = {hx, hy, hz};
+  double best = __builtin_inf();
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  for (int64_t f = (int64_t)blockIdx.x * VB + threadIdx.x; f < F; f += stride) {
+    const int64_t x = f % mx + 1, y = (f / mx) % my + 1, z = f / (mx * my) + 1;
+    const int64_t node = (z * ny + y) * nx + x;
+    const int64_t idx[3] = {x, y, z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double pc = p[c * F + f];
+      const double xc = (org[c] + h[c] * (double)idx[c]) + u[3 * node + c];
+      if (pc > 0.0) best = fmin(best, (hi[c] - xc) / pc);
+      else if (pc < 0.0) best = fmin(best, (lo[c] - xc) / pc);
+    }
+  }
+  __shared__ double sh[VB / 64];
+  best = wave_min(best);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+}
+
+// max |a - b| and max |b| (step_ratio, common.py: relative step of a candidate).
+__global__ void __launch_bounds__(VB) maxabs_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                    int64_t n, double* __restrict__ partials) {
+  double m0 = 0.0, m1 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += stride) {
+    const double d = a[i] - b[i];
+    m0 = (fabs(d) > m0 || d != d) ? fabs(d) : m0;   // NaN propagates
+    m1 = (fabs(b[i]) > m1 || b[i] != b[i]) ? fabs(b[i]) : m1;
+  }
+  __shared__ double sh[VB / 64][2];
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6][0] = m0;
+    sh[threadIdx.x >> 6][1] = m1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = fmax(fmax(sh[0][0], sh[1][0]), fmax(sh[2][0], sh[3][0]));
+    partials[2 * blockIdx.x + 1] = fmax(fmax(sh[0][1], sh[1][1]), fmax(sh[2][1], sh[3][1]));
+  }
+}
+
+__global__ void __launch_bounds__(64) fold_min_max_kernel(const double* __restrict__ partials, int nblocks, int mode,
+                                                          double* __restrict__ out) {
+  // mode 0: min of partials[b]; mode 1: max of partials[2b] and of partials[2b+1]
+  const int lane = threadIdx.x;
+  if (mode == 0) {
+    double m = __builtin_inf();
+    for (int b = lane; b < nblocks; b += 64) m = fmin(m, partials[b]);
+    m = wave_min(m);
+    if (lane == 0) out[0] = m;
+  } else {
+    double m0 = 0.0, m1 = 0.0;
+    for (int b = lane; b < nblocks; b += 64) {
+      m0 = fmax(m0, partials[2 * b]);
+      m1 = fmax(m1, partials[2 * b + 1]);
+    }
+    m0 = wave_max(m0);
+    m1 = wave_max(m1);
+    if (lane == 0) {
+      out[0] = m0;
+      out[1] = m1;
+    }
+  }
+}
+
 }  // namespace seqvp
 
 using namespace seqvp;
@@ -210,5 +330,40 @@ extern "C" int seqvp_metric_scale(const double* diag, const double* node, double
   hipLaunchKernelGGL(copy_or_zero_kernel, dim3(1), dim3(256), 0, s, nullptr, (double*)bad, 1);
   hipLaunchKernelGGL(metric_scale_kernel, dim3((F + VB - 1) / VB), dim3(VB), 0, s, diag, node, sqrt(m), act, floor_, nx,
                      ny, nz, scale, (unsigned long long*)bad);
+  return launch_status();
+}
+
+extern "C" int seqvp_bb2_products(const double* theta, const double* theta_old, const double* g, const double* g_old,
+                                  const double* p_old, const double* p, int64_t n, double* out, void* ws,
+                                  size_t ws_bytes, void* stream) {
+  if (!theta || !theta_old || !g || !g_old || !p_old || !p || !out || n < 0) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < seqvp_vec_workspace_bytes(n)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(bb2_kernel, dim3(VGRID), dim3(VB), 0, s, theta, theta_old, g, g_old, p_old, p, n, part);
+  hipLaunchKernelGGL(dots_reduce_kernel, dim3(1), dim3(VB), 0, s, part, VGRID, 4, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_step_bound(const double* p, const double* u, double ox, double oy, double oz, double hx,
+                                double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* out, void* ws,
+                                size_t ws_bytes, void* stream) {
+  if (!p || !u || !out || nx < 3 || ny < 3 || nz < 3) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < seqvp_vec_workspace_bytes(0)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(step_bound_kernel, dim3(VGRID), dim3(VB), 0, s, p, u, ox, oy, oz, hx, hy, hz, nx, ny, nz, part);
+  hipLaunchKernelGGL(fold_min_max_kernel, dim3(1), dim3(64), 0, s, part, VGRID, 0, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_max_abs_diff(const double* a, const double* b, int64_t n, double* out, void* ws, size_t ws_bytes,
+                                  void* stream) {
+  if (!a || !b || !out || n < 0) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < seqvp_vec_workspace_bytes(n)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(maxabs_kernel, dim3(VGRID), dim3(VB), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(fold_min_max_kernel, dim3(1), dim3(64), 0, s, part, VGRID, 1, out);
   return launch_status();
 }

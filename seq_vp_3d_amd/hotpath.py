@@ -159,8 +159,8 @@ class PairState:
         diag_nodes = q.diagonal_nodes(self.node_a, self.u)
         self.restrict(diag_nodes, self.diag)
         # alpha = factor * mean over free nodes of (q/omega) |b|^2 / 3 (q/omega = 1 on free nodes)
-        self.restrict(self.bgrad, self.g)
-        (bb,) = self.dots((self.g, self.g))
+        self.restrict(self.bgrad, self.p)          # p is free until direction(); g holds the gradient
+        (bb,) = self.dots((self.p, self.p))
         alpha = shift_factor * (bb / (self.nfree // 3)) / 3
         a = w.lam + alpha
         bp = w.mu / 2 + w.nu / 2
@@ -227,3 +227,77 @@ class PairState:
         self.warped, self.warped_t = self.warped_t, self.warped
         self.spline.warp(self.u, value=self.warped, gradient=self.bgrad)
         self.energy = energy
+
+    # ---- outer-loop support ----------------------------------------------------------------------
+    def snapshot(self):
+        """Device copies of the accepted state (the outer start)."""
+        return dict(theta=self.theta.clone(), u=self.u.clone(), warped=self.warped.clone(), bgrad=self.bgrad.clone(),
+                    energy=self.energy)
+
+    def restore(self, snap):
+        self.theta.copy_(snap["theta"])
+        self.u.copy_(snap["u"])
+        self.warped.copy_(snap["warped"])
+        self.bgrad.copy_(snap["bgrad"])
+        self.energy = snap["energy"]
+
+    def compare_to(self, snap):
+        """(E(current) - E(snap), F(current) - F(snap), min gap): stable differences of the current state
+        against a snapshot with the same reference (sequential.py:549-555; accelerated.py:2965-3060)."""
+        q, w = self.q1, self.problem.weights
+        node = q.node_energy(self.warped, self.fixed, self.u, self.ref, w.lam, data_weight=self.data_weight,
+                             base_warped=snap["warped"], base_u=snap["u"])
+        soft = q.soft_energy(self.u, self.ref, snap["u"])
+        bar = q.barrier_energy(self.u, snap["u"])
+        n, s_, b = node.tolist(), soft.tolist(), bar.tolist()
+        change = q.barrier_weight * b[2]
+        return n[2] + n[3] + s_[2] + change, n[2] + change, min(b[1], b[3]) - q.delta
+
+    def outer_norms(self):
+        """(GF, GD, GE) = mass norms of the outer (image + barrier), image-only and total gradients
+        (sequential.py:288-338; linearization.py:217-238)."""
+        q, w = self.q1, self.problem.weights
+        q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
+                     image=self.node_a, total=self.node_b)
+        outer, total = q.joint(self.node_a, self.node_b, self.u, self.ref)
+        gd = self.restrict(self.node_a, self.p)        # p, diag and g are free here (recomputed before use)
+        gf = self.restrict(outer, self.diag)
+        ge = self.restrict(total, self.g)
+        ff, dd, ee = self.dots((gf, gf), (gd, gd), (ge, ge))
+        r = float(np.sqrt(self.grid.omega))
+        return float(np.sqrt(ff)) / r, float(np.sqrt(dd)) / r, float(np.sqrt(ee)) / r
+
+    def refresh_reference(self):
+        """Self reference: u_ref <- u; the change terms of the state vanish exactly (problem.py:397-403)."""
+        self.ref.copy_(self.u)
+        self.energy = self.evaluate_energy(self.u, self.warped)
+
+    def gradient_norm(self):
+        (gg,) = self.dots((self.g, self.g))
+        return float(np.sqrt(gg)) / float(np.sqrt(self.grid.omega))
+
+    def bb2_products(self, prev):
+        """(s.y, |s|, |y|, y.iy) for the previous accepted (theta, g, p)."""
+        out = self.dot_out
+        _lib.call("seqvp_bb2_products", _lib.ptr(self.theta), _lib.ptr(prev["theta"]), _lib.ptr(self.g),
+                  _lib.ptr(prev["g"]), _lib.ptr(prev["p"]), _lib.ptr(self.p), self.nfree, _lib.ptr(out),
+                  _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        sy, ss, yy2, yiy = out.tolist()
+        return sy, float(np.sqrt(ss)), float(np.sqrt(yy2)), yiy
+
+    def step_bound(self):
+        nz, ny, nx = self.shape
+        hx, hy, hz = self.grid.spacing
+        ox, oy, oz = self.grid.origin
+        _lib.call("seqvp_step_bound", _lib.ptr(self.p), _lib.ptr(self.u), ox, oy, oz, hx, hy, hz, nx, ny, nz,
+                  _lib.ptr(self.dot_out), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        return float(self.dot_out[0].item())
+
+    def step_ratio(self, a, b):
+        """max|a - b| / max(L, max|b|), L = |axis lengths| (solvers/common.py:63-73)."""
+        _lib.call("seqvp_max_abs_diff", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.dot_out),
+                  _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        top, bmax = self.dot_out[:2].tolist()
+        length = float(np.linalg.norm([ax[-1] - ax[0] for ax in self.grid.axes]))
+        return float(np.float64(top) / max(length, bmax))
+

@@ -1,0 +1,298 @@
+"""Device-resident certified PG solver: the inner projected-gradient loop and the outer proximal loop.
+
+Restates the decisions of the reference's default route -- solve_poisson_gradient with
+ConservativePGConfig3D(resolution_exit=True, jd_step_global=True) and the
+image_shift_ggn_equilibrated_jd_affine metric (jd_curl/three_d/solvers/poisson_gradient.py:1090-1646), and
+solve_sequential (jd_curl/three_d/sequential.py:151-689) -- on a PairState whose arrays never leave HBM.
+Only scalars cross PCIe. The step bound is the T66 global closed27 root (no historical witness memory:
+its prediction is the proposal itself). A trial is valid only when every cell is proven J > delta by the
+GPU Bernstein prescreen and the closed27 samples are above delta; cells the prescreen cannot prove make
+the trial FEASIBILITY_UNCERTAIN (the authoritative CPU interval certificate for such cells is not part of
+this module).
+"""
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+import torch
+
+EPS = float(np.finfo(float).eps)
+
+
+@dataclass(frozen=True)
+class PGConfig:
+    """ConservativePGConfig3D defaults of the native level (poisson_gradient.py:23-235; usage.md:85-93)."""
+    max_iterations: int = 5
+    initial_step: float = 1.
+    step_min: float = 1e-8
+    step_max: float = 10.
+    max_backtracks: int = 25
+    c1: float = 1e-4
+    step_tol: float = 1e-12
+    stall_steps: int = 5
+    jd_step_safety: float = .9
+    resolution_exit: bool = True
+    jd_step_global: bool = True
+    shift_factor: float = 1.
+    jd_affine_activation: float = .25
+    fp32_direction: bool = False
+
+
+@dataclass(frozen=True)
+class SequentialConfig:
+    """SequentialReuseConfig3D (sequential.py:28-56) with the native run's max_iterations."""
+    max_iterations: int = 1000
+    eta0: float = .01
+    step_tol: float = 1e-12
+    stall_steps: int = 5
+
+
+@dataclass
+class InnerResult:
+    status: str
+    reason: str
+    accepted_steps: int
+    gradient_norm: float | None
+    trials: int
+    history: list = field(default_factory=list)
+
+
+@dataclass
+class SequentialResult:
+    status: str
+    reason: str
+    outer_accepted: int
+    G0: float
+    GF: float
+    threshold: float
+    history: list
+    seconds: float
+    energy: object = None
+
+
+def _protect(config, value):
+    return min(config.step_max, max(config.step_min, value))
+
+
+def step_proposal(st, config, previous):
+    """Accepted-only BB2 with the reference's fallbacks (poisson_gradient.py:955-990)."""
+    if previous is None:
+        return _protect(config, config.initial_step), "initial"
+    fallback = _protect(config, 2 * min(previous["tau"], config.step_max / 2))
+    sy, ns, ny, yiy = st.bb2_products(previous)
+    if not all(np.isfinite(v) for v in (sy, ns, ny, yiy)):
+        return fallback, "fallback_nonfinite_secant"
+    if sy <= 0:
+        return fallback, "fallback_nonpositive_curvature"
+    cosine = sy / (ns * ny) if ns > 0 and ny > 0 else 0.
+    if not np.isfinite(cosine) or cosine <= 1e-12:
+        return fallback, "fallback_small_curvature"
+    if yiy <= 0:
+        return fallback, "fallback_nonpositive_metric_curvature"
+    proposal = sy / yiy
+    if not np.isfinite(proposal) or proposal <= 0:
+        return fallback, "fallback_nonpositive_step"
+    return _protect(config, proposal), "bb2"
+
+
+def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=None):
+    """One frozen-reference inner solve on the accepted state of `st` (poisson_gradient.py:1090-1646)."""
+    history = []
+    accepted_steps, small_steps, trials = 0, 0, 0
+    previous = None
+    metric_ready = False
+    G = None
+
+    def finish(status, reason):
+        return InnerResult(status, reason, accepted_steps, G, trials, history)
+
+    while True:
+        try:
+            st.gradient()
+            G = st.gradient_norm()
+        except FloatingPointError as exc:
+            return finish("nonfinite_evaluation", str(exc))
+        if not np.isfinite(G):
+            return finish("nonfinite_evaluation", "nonfinite gradient")
+        if G <= target:
+            return finish("converged_gradient", "GRADIENT_TARGET")
+        if budget is not None and budget():
+            return finish("work_budget_exhausted", "WALL_BUDGET")
+        if small_steps >= config.stall_steps:
+            return finish("stagnated", "SMALL_ACCEPTED_STEPS")
+        if accepted_steps >= config.max_iterations:
+            return finish("max_iterations", "ACCEPTED_STEP_BUDGET")
+        if not metric_ready:
+            try:
+                st.metric(shift_factor=config.shift_factor, activation=config.jd_affine_activation)
+            except FloatingPointError as exc:
+                return finish("nonfinite_evaluation", str(exc))
+            metric_ready = True
+        st.direction(fp32=config.fp32_direction)
+        (slope,) = st.dots((st.g, st.p))
+        if not np.isfinite(slope):
+            return finish("nonfinite_evaluation", "PG directional derivative")
+        if slope >= 0:
+            return finish("non_descent_direction", "NON_DESCENT_DIRECTION")
+        proposal, rule = step_proposal(st, config, previous)
+        bound = st.step_bound()
+        if bound <= 0:
+            return finish("domain_blocked", "NO_POSITIVE_DOMAIN_STEP")
+        tau = min(proposal, .9 * bound)
+        untruncated = tau == proposal
+        root_info = None
+        if config.jd_step_global:
+            try:
+                root_s, invalid, points = st.jd_bound(tau)
+            except FloatingPointError as exc:
+                root_s, invalid, points = float("nan"), 0, 0
+                root_info = dict(fallback=str(exc))
+            if root_info is None:
+                root = None if not np.isfinite(root_s) else root_s * tau
+                root_info = dict(limit=tau, root=root, invalid_base_points=invalid, points=points)
+                if root == 0:
+                    root_info["fallback"] = "zero_root_below_resolution"
+                elif root is not None:
+                    tau = min(tau, config.jd_step_safety * root)
+        exit_floor = 15 * EPS * st.energy.total if config.resolution_exit else None
+        global_attempts = 0
+        accepted = False
+        for attempt in range(config.max_backtracks + 1):
+            if budget is not None and budget():
+                return finish("work_budget_exhausted", "WALL_BUDGET")
+            trials += 1
+            tr = st.trial(tau)
+            required = -config.c1 * tau * slope
+            actual = resolution = certified = None
+            if not tr.ok:
+                reason = tr.reason
+            elif tr.unproven_cells:
+                reason = "FEASIBILITY_UNCERTAIN"
+            else:
+                actual = -tr.total_difference
+                resolution = 16 * EPS * max(st.energy.total, tr.energy.total)
+                certified = actual - resolution
+                if not np.isfinite(actual):
+                    reason = "NONFINITE_COMPARISON"
+                else:
+                    accepted = certified > 0 and certified >= required
+                    reason = ("ACCEPTED" if accepted else "ENERGY_RESOLUTION" if abs(actual) <= resolution
+                              else "NO_STRICT_DECREASE" if actual < 0 else "ARMIJO_REJECTED")
+            exit_info = None
+            if (config.resolution_exit and not accepted and not reason.startswith("NONFINITE")
+                    and global_attempts + 1 <= config.max_backtracks):
+                next_tau = .5 * tau
+                decrease = next_tau * abs(slope)
+                exit_info = dict(next_tau=next_tau, predicted_decrease=decrease, resolution_floor=exit_floor,
+                                 stop=bool(decrease <= exit_floor))
+            row = dict(iteration=accepted_steps, attempt=attempt + 1, accepted=bool(accepted), reason=reason, tau=tau,
+                       step_proposal=proposal, step_rule=rule, untruncated=untruncated, domain_step_bound=bound,
+                       gradient_norm=G, gradient_dot_direction=slope, actual_reduction=actual,
+                       energy_resolution=resolution, certified_reduction=certified, energy_base=st.energy.total,
+                       energy_trial=tr.energy.total if tr.energy is not None else None,
+                       unproven_cells=tr.unproven_cells, jd_root=root_info, resolution_exit=exit_info)
+            history.append(row)
+            if observer is not None:
+                observer(row)
+            if reason.startswith("NONFINITE"):
+                return finish("nonfinite_evaluation", reason)
+            if accepted:
+                old = dict(theta=st.theta.clone(), g=st.g.clone(), p=st.p.clone(), tau=tau)
+                relative = st.step_ratio(st.theta_t, st.theta)
+                small_steps = small_steps + 1 if relative <= config.step_tol else 0
+                st.promote(tr.energy)
+                previous = old
+                accepted_steps += 1
+                break
+            global_attempts += 1
+            if global_attempts > config.max_backtracks:
+                break
+            if exit_info is not None:
+                tau = exit_info["next_tau"]
+                if exit_info["stop"]:
+                    break
+            else:
+                tau *= .5
+        if not accepted:
+            last = history[-1]["reason"]
+            status = ("precision_stagnated" if last in ("ENERGY_RESOLUTION", "NO_STRICT_DECREASE")
+                      else "feasibility_uncertain" if last == "FEASIBILITY_UNCERTAIN"
+                      else "trial_budget_exhausted")
+            return finish(status, last)
+
+
+def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), budget=None, observer=None,
+                     max_outer=None):
+    """Outer proximal loop (sequential.py:441-689): inner solve from the committed state, stable E and F
+    comparisons against the start, commit, self-reference refresh."""
+    started = perf_counter()
+    history = []
+    st.refresh_reference()
+    GF, GD, GE = st.outer_norms()
+    G0 = GF
+    threshold = 1e-8 + 1e-6 * G0
+    accepted_count, small_steps = 0, 0
+    limit = config.max_iterations if max_outer is None else min(config.max_iterations, max_outer)
+
+    def finish(status, reason):
+        return SequentialResult(status, reason, accepted_count, G0, GF, threshold, history,
+                                perf_counter() - started, st.energy)
+
+    while True:
+        if budget is not None and budget():
+            return finish("outer_time_budget", "WALL_BUDGET")
+        if GF <= threshold:
+            return finish("converged_regularized_gradient", "Feasible F=D+B gradient criterion")
+        if small_steps >= config.stall_steps:
+            return finish("outer_stagnated", "Small accepted steps with nonzero outer gradient")
+        if accepted_count >= limit:
+            return finish("outer_max_iterations", "Outer budget exhausted; not convergence")
+        tick = perf_counter()
+        start = st.snapshot()
+        eta = config.eta0 if G0 == 0 else config.eta0 * np.sqrt(min(GF, G0) / G0)
+        target = float(eta * GF)
+        inner = solve_poisson_gradient(st, target, inner_config, budget=budget)
+        eligible = inner.status in ("converged_gradient", "max_iterations", "stagnated")
+        accepted, reason = False, "INNER_FAILED_NOT_COMMITTED"
+        total = fixed = None
+        GFa = GDa = GEa = relative = None
+        if inner.accepted_steps > 0:
+            GFa, GDa, GEa = st.outer_norms()
+            relative = st.step_ratio(st.theta, start["theta"])
+            dE, dF, gap = st.compare_to(start)
+            resE = 16 * EPS * max(start["energy"].total, st.energy.total)
+            resF = 16 * EPS * max(start["energy"].outer, st.energy.outer)
+            total, fixed = (-dE, resE), (-dF, resF)
+            cert = (-dE - resE) > 0 and (-dF - resF) > 0
+            same = bool(torch.equal(st.theta, start["theta"]))
+            accepted = bool(eligible and cert and not same)
+            reason = ("ACCEPTED" if accepted else "INNER_FAILED_NOT_COMMITTED" if not eligible
+                      else "NO_PROGRESS" if same else "DESCENT_CERTIFICATE_FAILED")
+        row = dict(outer_i=len(history), E_start=start["energy"].total, F_start=start["energy"].outer,
+                   E_candidate=st.energy.total if inner.accepted_steps else None,
+                   actual_reduction_E=None if total is None else total[0],
+                   resolution_E=None if total is None else total[1],
+                   actual_reduction_F=None if fixed is None else fixed[0],
+                   resolution_F=None if fixed is None else fixed[1],
+                   GF_before=GF, GF_candidate=GFa, GD_candidate=GDa, GE_candidate=GEa, forcing_eta=float(eta),
+                   forcing_target=target, inner_status=inner.status, inner_reason=inner.reason,
+                   inner_accepted_steps=inner.accepted_steps, inner_trials=inner.trials, outer_accepted=accepted,
+                   reason=reason, step_ratio=relative, seconds=perf_counter() - tick)
+        history.append(row)
+        if observer is not None:
+            observer(row)
+        if not accepted:
+            st.restore(start)
+            if inner.status == "precision_stagnated":
+                return finish("outer_precision_stagnated", reason)
+            if inner.status in ("domain_blocked", "feasibility_uncertain"):
+                return finish("outer_" + inner.status, inner.reason)
+            if not eligible:
+                return finish("outer_inner_failed", inner.status + ": " + inner.reason)
+            return finish("outer_no_progress" if reason == "NO_PROGRESS" else "outer_candidate_rejected", reason)
+        accepted_count += 1
+        GF, GD, GE = GFa, GDa, GEa
+        small_steps = small_steps + 1 if relative <= config.step_tol else 0
+        if GF <= threshold or accepted_count >= limit or small_steps >= config.stall_steps:
+            continue
+        st.refresh_reference()

@@ -1,0 +1,41 @@
+"""The device solver against the oracle-backed CPU twin, step by step (same decisions, same tau)."""
+import numpy as np
+import pytest
+
+from cpu_pair import CpuPairState
+from oracle import q1 as Q, spline as osp
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def test_device_solver_matches_cpu_twin_stepwise(gpu_device):
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print("case solver stepwise 14x15x16", flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    spec = PairSpec((14, 15, 16), 6, (2., 4.), 3, .6, .01, 3)
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    cpu = CpuPairState(spec.shape, (1., 1., 1.), w, pair.fixed.numpy(), pair.moving.numpy())
+    theta0 = Q.restrict_free(.5 * pair.displacement.numpy(), cpu.grid)
+    cpu.set_state(theta0)
+    gpu = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+    gpu.set_state(theta=torch.tensor(theta0, device=gpu_device))
+    rows_c, rows_g = [], []
+    rc = solve_sequential(cpu, SequentialConfig(), PGConfig(), max_outer=2, observer=rows_c.append)
+    rg = solve_sequential(gpu, SequentialConfig(), PGConfig(), max_outer=2, observer=rows_g.append)
+    assert rc.status == rg.status and rc.outer_accepted == rg.outer_accepted == 2
+    for a, b in zip(rows_c, rows_g):
+        assert a["reason"] == b["reason"] and a["inner_accepted_steps"] == b["inner_accepted_steps"]
+        assert a["inner_trials"] == b["inner_trials"]
+        for key in ("E_start", "E_candidate", "GF_candidate", "GE_candidate"):
+            assert abs(a[key] - b[key]) <= 1e-10 * abs(a[key]), key
+    np.testing.assert_allclose(gpu.theta.cpu().numpy(), cpu.theta.numpy(), rtol=0,
+                               atol=1e-10 * np.abs(cpu.theta.numpy()).max())

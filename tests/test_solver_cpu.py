@@ -1,0 +1,73 @@
+"""The device solver's decision logic (seq_vp_3d_amd/solver.py) run on the oracle-backed CPU twin."""
+import numpy as np
+import pytest
+import torch
+
+from cpu_pair import CpuPairState
+from oracle import q1 as Q, spline as osp
+from seq_vp_3d_amd.problem import native_weights
+from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_poisson_gradient, solve_sequential, step_proposal
+from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+
+
+def _sampler(img, grid, pts):
+    s = osp.build(img.numpy(), Q.Grid(grid.shape, grid.spacing))
+    return torch.tensor(osp.sample(s, pts.numpy(), with_gradient=False)[0])
+
+
+@pytest.fixture(scope="module")
+def pair():
+    spec = PairSpec((14, 15, 16), 6, (2., 4.), 3, .6, .01, 3)
+    return spec, make_pair(spec, "cpu", _sampler)
+
+
+def _state(spec, pair):
+    st = CpuPairState(spec.shape, (1., 1., 1.), native_weights(spec.shape), pair.fixed.numpy(), pair.moving.numpy())
+    st.set_state(Q.restrict_free(.5 * pair.displacement.numpy(), st.grid))
+    return st
+
+
+def test_outer_loop_commits_and_decreases_energy(pair):
+    spec, p = pair
+    st = _state(spec, p)
+    rows = []
+    res = solve_sequential(st, SequentialConfig(), PGConfig(), max_outer=2, observer=rows.append)
+    assert res.outer_accepted == 2 and res.status == "outer_max_iterations"
+    for r in rows:
+        assert r["reason"] == "ACCEPTED" and r["actual_reduction_E"] > r["resolution_E"]
+        assert r["actual_reduction_F"] > r["resolution_F"] and r["inner_accepted_steps"] >= 1
+    assert res.GF < res.G0
+
+
+def test_inner_attempt_rows_follow_the_acceptance_rule(pair):
+    spec, p = pair
+    st = _state(spec, p)
+    st.refresh_reference()
+    rows = []
+    inner = solve_poisson_gradient(st, 0., PGConfig(max_iterations=3), observer=rows.append)
+    assert inner.accepted_steps == 3 and inner.status == "max_iterations"
+    for r in rows:
+        if r["accepted"]:
+            assert r["certified_reduction"] > 0 and r["certified_reduction"] >= -1e-4 * r["tau"] * r["gradient_dot_direction"]
+        assert r["tau"] <= .9 * r["domain_step_bound"] + 1e-300
+        if r["jd_root"] and r["jd_root"]["root"] is not None:
+            assert r["tau"] <= .9 * r["jd_root"]["root"] * (1 + 1e-15) or r["tau"] < r["jd_root"]["limit"]
+    assert rows[0]["step_rule"] == "initial" and all(r["step_rule"] != "initial" for r in rows[1:])
+
+
+def test_bb2_fallbacks():
+    class S:
+        def __init__(self, vals):
+            self.vals = vals
+
+        def bb2_products(self, prev):
+            return self.vals
+
+    cfg = PGConfig()
+    prev = dict(tau=.3)
+    assert step_proposal(S((1., 1., 1., 2.)), cfg, prev) == (.5, "bb2")
+    assert step_proposal(S((-1., 1., 1., 2.)), cfg, prev) == (.6, "fallback_nonpositive_curvature")
+    assert step_proposal(S((1e-20, 1., 1., 2.)), cfg, prev) == (.6, "fallback_small_curvature")
+    assert step_proposal(S((1., 1., 1., -2.)), cfg, prev) == (.6, "fallback_nonpositive_metric_curvature")
+    assert step_proposal(S((1., 1., 1., 1e-12)), cfg, prev) == (10., "bb2")
+    assert step_proposal(S((1., 1., 1., 1.)), cfg, None) == (1., "initial")

@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--shape", type=str, default="160,192,224")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("driver", "fixed"), default="driver")
+    ap.add_argument("--fp32-screen", action="store_true", help="precision form: FP32 screening energies")
     return ap.parse_args()
 
 
@@ -142,7 +143,7 @@ def main():
                 print(f"{label} {k} done E {st.energy.total:.6e}", file=sys.stderr, flush=True)
     else:
         from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
-        inner_cfg = PGConfig(max_iterations=args.inner)
+        inner_cfg = PGConfig(max_iterations=args.inner, fp32_screen=args.fp32_screen)
 
         def run(n, label):
             def obs(row):

@@ -52,6 +52,16 @@ int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base,
 int seqvp_q1_barrier_energy(const double* u, const double* base, const double* k, int64_t nx, int64_t ny,
                             int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* FP32 screening variants of the two energies above (replace FusedQ1.soft_energy_f32 / barrier_energy_f32,
+ * triton_q1.py:652-678; accelerated.py:1603-1612): node values rounded to binary32 on load, per-point
+ * terms in binary32, per-lane Kahan-compensated binary32 sums, block/grid sums in double-double.
+ * Same outputs and workspace as the FP64 entry points. */
+int seqvp_q1_soft_energy_f32(const double* u, const double* ref, const double* base, const double* k, int64_t nx,
+                             int64_t ny, int64_t nz, double* out, void* workspace, size_t workspace_bytes,
+                             void* stream);
+int seqvp_q1_barrier_energy_f32(const double* u, const double* base, const double* k, int64_t nx, int64_t ny,
+                                int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Node-term energies (replaces the image/proximity part of AcceleratedHybridBackend._energy,
  * accelerated.py:1583-1600; residuals.py:189-195, energy_comparison.py:196-210).
  * q is the trapezoid node mass omega*tz*ty*tx computed in-kernel; qd = q*data_weight (data_weight may

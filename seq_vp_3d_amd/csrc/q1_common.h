@@ -59,22 +59,26 @@ __host__ __device__ constexpr int edge_a0(int j, int k) {
 }
 __host__ __device__ constexpr int edge_a1(int j, int k) { return edge_a0(j, k) | (1 << j); }
 
-struct Edges {
-  double e[3][3][4];   // [j][r][k]
+template <typename T = double>
+struct EdgesT {
+  T e[3][3][4];   // [j][r][k]
 };
+typedef EdgesT<double> Edges;
 
-// Loads the 8 vertices (24 doubles) of a cell.
-__device__ __forceinline__ void load_cell(const double* __restrict__ f, int64_t b, const Dims& d, double v[8][3]) {
+// Loads the 8 vertices (24 values) of a cell, rounding to T.
+template <typename T>
+__device__ __forceinline__ void load_cell(const double* __restrict__ f, int64_t b, const Dims& d, T v[8][3]) {
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const double* p = f + 3 * vertex_node(b, a, d);
-    v[a][0] = p[0];
-    v[a][1] = p[1];
-    v[a][2] = p[2];
+    v[a][0] = (T)p[0];
+    v[a][1] = (T)p[1];
+    v[a][2] = (T)p[2];
   }
 }
 
-__device__ __forceinline__ void make_edges(const double v[8][3], const double ih[3], Edges& E) {
+template <typename T>
+__device__ __forceinline__ void make_edges(const T v[8][3], const T ih[3], EdgesT<T>& E) {
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -85,12 +89,13 @@ __device__ __forceinline__ void make_edges(const double v[8][3], const double ih
 }
 
 // V[r][j] = sum_k e[j][r][k] f(k0, s_o1) f(k1, s_o2) at reference point s = (sx, sy, sz).
-__device__ __forceinline__ void point_grad(const Edges& E, const double s[3], double V[3][3]) {
+template <typename T>
+__device__ __forceinline__ void point_grad(const EdgesT<T>& E, const T s[3], T V[3][3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const double so1 = s[other1(j)], so2 = s[other2(j)];
-    const double w00 = (1.0 - so1) * (1.0 - so2), w10 = so1 * (1.0 - so2);
-    const double w01 = (1.0 - so1) * so2, w11 = so1 * so2;
+    const T so1 = s[other1(j)], so2 = s[other2(j)];
+    const T w00 = ((T)1 - so1) * ((T)1 - so2), w10 = so1 * ((T)1 - so2);
+    const T w01 = ((T)1 - so1) * so2, w11 = so1 * so2;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
       V[r][j] = (E.e[j][r][0] * w00 + E.e[j][r][1] * w10) + (E.e[j][r][2] * w01 + E.e[j][r][3] * w11);
@@ -98,11 +103,12 @@ __device__ __forceinline__ void point_grad(const Edges& E, const double s[3], do
 }
 
 // F = I + V; cofactor rows A_r = F_{r+1} x F_{r+2}; J = F_0 . A_0 (geometry.py:11-23, 60-89).
-__device__ __forceinline__ double jacobian(const double V[3][3]) {
-  const double f00 = V[0][0] + 1.0, f11 = V[1][1] + 1.0, f22 = V[2][2] + 1.0;
-  const double a00 = f11 * f22 - V[1][2] * V[2][1];
-  const double a01 = V[1][2] * V[2][0] - V[1][0] * f22;
-  const double a02 = V[1][0] * V[2][1] - f11 * V[2][0];
+template <typename T>
+__device__ __forceinline__ T jacobian(const T V[3][3]) {
+  const T f00 = V[0][0] + (T)1, f11 = V[1][1] + (T)1, f22 = V[2][2] + (T)1;
+  const T a00 = f11 * f22 - V[1][2] * V[2][1];
+  const T a01 = V[1][2] * V[2][0] - V[1][0] * f22;
+  const T a02 = V[1][0] * V[2][1] - f11 * V[2][0];
   return (f00 * a00 + V[0][1] * a01) + V[0][2] * a02;
 }
 
@@ -124,7 +130,8 @@ __device__ __forceinline__ double cofactor_jacobian(const double V[3][3], double
   return (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
 }
 
-__device__ __forceinline__ void curl_of(const double V[3][3], double C[3]) {
+template <typename T>
+__device__ __forceinline__ void curl_of(const T V[3][3], T C[3]) {
   C[0] = V[2][1] - V[1][2];
   C[1] = V[0][2] - V[2][0];
   C[2] = V[1][0] - V[0][1];
@@ -207,11 +214,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nblocks) {
 
 // ---- localized C2 barrier (barrier.py:19-36; triton_q1_kernels _barrier_*) ----------------------
 // r = max(act - J, 0)/scale, t = (min(J, act) - delta)/scale; log t through log1p(-r) when r < 1/2.
-__device__ __forceinline__ double barrier_logt(double r, double t) { return r < 0.5 ? log1p(-r) : log(t); }
+template <typename T>
+__device__ __forceinline__ T barrier_logt(T r, T t) { return r < (T)0.5 ? log1p(-r) : log(t); }
 
-__device__ __forceinline__ double barrier_value(double J, double w, double delta, double act, double scale) {
-  if (!(J < act)) return 0.0;
-  const double r = (act - J) / scale, t = (J - delta) / scale;
+template <typename T>
+__device__ __forceinline__ T barrier_value(T J, T w, T delta, T act, T scale) {
+  if (!(J < act)) return (T)0;
+  const T r = (act - J) / scale, t = (J - delta) / scale;
   return -w * r * r * barrier_logt(r, t);
 }
 
@@ -231,15 +240,28 @@ __device__ __forceinline__ double barrier_second(double J, double w, double delt
 }
 
 // Unweighted stable psi(Jb) - psi(Ja) (energy_comparison.py:14-46, localized branch).
-__device__ __forceinline__ double barrier_change(double Ja, double Jb, double delta, double act, double scale) {
-  const double a = fmin(Ja, act), b = fmin(Jb, act);
-  if (a == act && b == act) return 0.0;
-  const double ra = (act - a) / scale, rb = (act - b) / scale;
-  const double h = (b - a) / scale;
-  const double loga = ra < 0.5 ? log1p(-ra) : log((a - delta) / scale);
-  const double x = (b - a) / (a - delta);
-  const double lr = x >= -0.5 ? log1p(x) : log((b - delta) / (a - delta));
+template <typename T>
+__device__ __forceinline__ T barrier_change(T Ja, T Jb, T delta, T act, T scale) {
+  const T a = fmin(Ja, act), b = fmin(Jb, act);
+  if (a == act && b == act) return (T)0;
+  const T ra = (act - a) / scale, rb = (act - b) / scale;
+  const T h = (b - a) / scale;
+  const T loga = ra < (T)0.5 ? log1p(-ra) : log((a - delta) / scale);
+  const T x = (b - a) / (a - delta);
+  const T lr = x >= (T)-0.5 ? log1p(x) : log((b - delta) / (a - delta));
   return h * (ra + rb) * loga - rb * rb * lr;
+}
+
+// Kahan-compensated single-precision accumulator (the FP32 screening sums).
+struct KahanF {
+  float s, c;
+};
+
+__device__ __forceinline__ void kahan_add(KahanF& k, float v) {
+  const float y = v - k.c;
+  const float t = k.s + y;
+  k.c = (t - k.s) - y;
+  k.s = t;
 }
 
 }  // namespace seqvp

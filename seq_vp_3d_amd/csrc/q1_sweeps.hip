@@ -213,6 +213,124 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
 }
 
 // ------------------------------------------------------------------------------------------------
+// FP32 screening energies (the precision form; triton_q1_f32_kernels.py's soft_energy_f32 /
+// barrier_energy_f32 roles): node values rounded to binary32 on load, every per-point term in binary32,
+// per-lane Kahan-compensated binary32 sums, block and grid passes in double-double. Same outputs as the
+// FP64 kernels; the error band of the screen is FP32_EPS*(64 (jd+curl) + 4096 barrier).
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK) soft_energy_f32_kernel(const double* __restrict__ u,
+                                                                const double* __restrict__ ref,
+                                                                const double* __restrict__ base,
+                                                                const double* __restrict__ k, Dims d,
+                                                                double* __restrict__ partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  if (c < d.ncell) {
+    const float ih[3] = {(float)k[K_IH], (float)k[K_IH + 1], (float)k[K_IH + 2]};
+    const float mu = (float)k[K_MU], nu = (float)k[K_NU];
+    const int64_t b = cell_base_node(c, d);
+    EdgesT<float> Eu, Er, Eb;
+    {
+      float v[8][3];
+      load_cell(u, b, d, v);
+      make_edges(v, ih, Eu);
+      load_cell(ref, b, d, v);
+      make_edges(v, ih, Er);
+      if (WITH_BASE) {
+        load_cell(base, b, d, v);
+        make_edges(v, ih, Eb);
+      }
+    }
+    KahanF cj{0.f, 0.f}, cc{0.f, 0.f}, cd{0.f, 0.f};
+#pragma unroll 1
+    for (int p = 0; p < 27; ++p) {
+      const float sp[3] = {(float)GAUSS_S[p % 3], (float)GAUSS_S[(p / 3) % 3], (float)GAUSS_S[p / 9]};
+      const float qw = (float)k[K_QW + p];
+      float V[3][3], W[3][3], C[3], Cr[3];
+      point_grad(Eu, sp, V);
+      const float J = jacobian(V);
+      curl_of(V, C);
+      point_grad(Er, sp, W);
+      const float Jr = jacobian(W);
+      curl_of(W, Cr);
+      const float dj = J - Jr;
+      const float d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
+      kahan_add(cj, qw * dj * dj);
+      kahan_add(cc, (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2);
+      if (WITH_BASE) {
+        float Y[3][3], Ca[3];
+        point_grad(Eb, sp, Y);
+        const float Ja = jacobian(Y);
+        curl_of(Y, Ca);
+        const float sm = sqrtf(mu * qw), sn = sqrtf(nu * qw);
+        const float xj = sm * (Ja - Jr), yj = sm * dj;
+        kahan_add(cd, 0.5f * ((yj - xj) * (yj + xj)));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const float xc = sn * (Ca[i] - Cr[i]), yc = sn * (C[i] - Cr[i]);
+          kahan_add(cd, 0.5f * ((yc - xc) * (yc + xc)));
+        }
+      }
+    }
+    dd_add(s[0], 0.5 * (double)mu * ((double)cj.s - (double)cj.c));
+    dd_add(s[1], 0.5 * (double)nu * ((double)cc.s - (double)cc.c));
+    dd_add(s[2], (double)cd.s - (double)cd.c);
+  }
+  block_reduce_store<3, 0>(s, none, partials + blk * 6);
+}
+
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK) barrier_energy_f32_kernel(const double* __restrict__ u,
+                                                                   const double* __restrict__ base,
+                                                                   const double* __restrict__ k, Dims d,
+                                                                   double* __restrict__ partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double m[2] = {__builtin_inf(), __builtin_inf()};
+  if (c < d.ncell) {
+    const float ih[3] = {(float)k[K_IH], (float)k[K_IH + 1], (float)k[K_IH + 2]};
+    const float w = (float)k[K_W], delta = (float)k[K_DELTA], act = (float)k[K_ACT], scale = (float)k[K_SCALE];
+    const int64_t b = cell_base_node(c, d);
+    EdgesT<float> Eu, Eb;
+    {
+      float v[8][3];
+      load_cell(u, b, d, v);
+      make_edges(v, ih, Eu);
+      if (WITH_BASE) {
+        load_cell(base, b, d, v);
+        make_edges(v, ih, Eb);
+      }
+    }
+    KahanF val{0.f, 0.f}, chg{0.f, 0.f};
+    float m0 = __builtin_inff(), m1 = __builtin_inff();
+#pragma unroll 1
+    for (int p = 0; p < 27; ++p) {
+      const float sp[3] = {(float)CLOSED_S[p % 3], (float)CLOSED_S[(p / 3) % 3], (float)CLOSED_S[p / 9]};
+      float V[3][3];
+      point_grad(Eu, sp, V);
+      const float J = jacobian(V);
+      m0 = (J != J || m0 != m0) ? __builtin_nanf("") : (J < m0 ? J : m0);
+      kahan_add(val, barrier_value(J, w, delta, act, scale));
+      if (WITH_BASE) {
+        float Y[3][3];
+        point_grad(Eb, sp, Y);
+        const float Ja = jacobian(Y);
+        m1 = (Ja != Ja || m1 != m1) ? __builtin_nanf("") : (Ja < m1 ? Ja : m1);
+        if (!(J >= act && Ja >= act)) kahan_add(chg, barrier_change(Ja, J, delta, act, scale));
+      }
+    }
+    dd_add(s[0], (double)val.s - (double)val.c);
+    dd_add(s[1], (double)chg.s - (double)chg.c);
+    m[0] = (double)m0;
+    m[1] = (double)m1;
+  }
+  block_reduce_store<2, 2>(s, m, partials + blk * 6);
+}
+
+// ------------------------------------------------------------------------------------------------
 // Per-cell vertex contributions (SoA: loc[(a*3+r)*ncell + c]).
 //
 // Gauss soft VJP: flux X = A*jd + curl adjoint with jd = mu qw (J-Jr), curl = nu qw (c-cr)
@@ -306,8 +424,12 @@ __device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __res
   for (int p = 0; p < 27; ++p) {
     double sp[3];
     closed_point(p, sp);
-    double V[3][3], A[3][3];
+    double V[3][3];
     point_grad(Eu, sp, V);
+    // psi'(J) is exactly 0 at and above the activation: a point where every lane of the wave is
+    // inactive contributes nothing and is skipped (wave-uniform branch).
+    if (!__any(jacobian(V) < act)) continue;
+    double A[3][3];
     const double J = cofactor_jacobian(V, A);
     const double b1 = barrier_first(J, w, delta, act, scale);
     double X[3][3];
@@ -950,5 +1072,39 @@ extern "C" int seqvp_node_terms(const double* warped, const double* fixed, const
   const Dims d = make_dims(nx, ny, nz);
   hipLaunchKernelGGL(node_terms_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, warped,
                      fixed, data_weight, image_grad, u, ref, lam, hx * hy * hz, d, image, total, ggn);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_soft_energy_f32(const double* u, const double* ref, const double* base, const double* k,
+                                        int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                        void* stream) {
+  if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(0, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  if (base)
+    hipLaunchKernelGGL((soft_energy_f32_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
+  else
+    hipLaunchKernelGGL((soft_energy_f32_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_barrier_energy_f32(const double* u, const double* base, const double* k, int64_t nx,
+                                           int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                           void* stream) {
+  if (!u || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(1, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  if (base)
+    hipLaunchKernelGGL((barrier_energy_f32_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, base, k, d, part);
+  else
+    hipLaunchKernelGGL((barrier_energy_f32_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, part);
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
   return launch_status();
 }

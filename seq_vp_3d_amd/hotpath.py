@@ -31,6 +31,11 @@ from .spline import DeviceSpline, build_image_sampler
 
 JD_AFFINE_FLOOR = 1e-12
 JD_AFFINE_ACTIVATION = .25
+# FP32 screening band per term, relative to that term's values of the two states compared
+# (accelerated.py:67-76): 64 ulps of binary32 for the JD/curl penalties, 4096 for the barrier.
+FP32_EPS = 2. ** -24
+FP32_BAND_SOFT = 64.
+FP32_BAND_BARRIER = 4096.
 
 
 @dataclass
@@ -59,6 +64,7 @@ class TrialResult:
     outer_difference: float = float("nan")     # F(trial) - F(base), stable
     min_gap: float = float("nan")
     unproven_cells: int = 0
+    band: float | None = None                  # FP32 screening band of the differences (None: FP64 values)
 
 
 class PairState:
@@ -195,19 +201,31 @@ class PairState:
         return self.root(self.u, self.v, limit)
 
     # ---- trial and promotion -----------------------------------------------------------------------
-    def trial(self, tau):
-        """Candidate theta + tau p: device prescreen, sampling and energies against the accepted state."""
+    def trial(self, tau, screen=False):
+        """Candidate theta + tau p: device prescreen, sampling and energies against the accepted state.
+        screen=True evaluates the soft and barrier terms with the FP32 screening kernels and returns the
+        screening band of the differences (accelerated.py:1603-1612, 67-76)."""
         nz, ny, nx = self.shape
-        q, w = self.q1, self.problem.weights
+        q = self.q1
         _lib.call("seqvp_axpy_inject", _lib.ptr(self.theta), _lib.ptr(self.p), float(tau), nx, ny, nz,
                   _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
         prescreen_flags(self.u_t, self.grid, self.D, q.delta, self.flags, self.count)
         _, _, bad = self.spline.warp(self.u_t, with_gradient=False, value=self.warped_t)
+        self._trial_bad = bad
+        return self.trial_energies(screen)
+
+    def trial_energies(self, screen=False):
+        """Energies and stable differences of the current candidate (u_t, warped_t) against the accepted state."""
+        q, w = self.q1, self.problem.weights
         node = q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
                              base_warped=self.warped, base_u=self.u)
-        soft = q.soft_energy(self.u_t, self.ref, self.u)
-        bar = q.barrier_energy(self.u_t, self.u)
-        vals = torch.cat((node, soft, bar, bad.to(torch.float64), self.count.to(torch.float64))).tolist()
+        if screen:
+            soft = q.soft_energy_f32(self.u_t, self.ref, self.u)
+            bar = q.barrier_energy_f32(self.u_t, self.u)
+        else:
+            soft = q.soft_energy(self.u_t, self.ref, self.u)
+            bar = q.barrier_energy(self.u_t, self.u)
+        vals = torch.cat((node, soft, bar, self._trial_bad.to(torch.float64), self.count.to(torch.float64))).tolist()
         n, s, b, nbad, unproven = vals[0:4], vals[4:7], vals[7:11], vals[11], int(vals[12])
         if nbad:
             return TrialResult(False, "INVALID_DOMAIN", None, unproven_cells=unproven)
@@ -218,7 +236,13 @@ class PairState:
         total = n[2] + n[3] + s[2] + change
         outer = n[2] + change
         gap = min(b[1], b[3]) - q.delta
-        return TrialResult(True, "", energy, total, outer, gap, unproven)
+        band = None
+        if screen:
+            base = self.energy
+            soft_sum = max(energy.jd + energy.curl + base.jd + base.curl, 0.)
+            bar_sum = max(energy.barrier + base.barrier, 0.)
+            band = FP32_EPS * (FP32_BAND_SOFT * soft_sum + FP32_BAND_BARRIER * bar_sum)
+        return TrialResult(True, "", energy, total, outer, gap, unproven, band)
 
     def promote(self, energy):
         """Make the trial the accepted state (swap buffers) and sample the image gradient there."""
@@ -268,9 +292,12 @@ class PairState:
         return float(np.sqrt(ff)) / r, float(np.sqrt(dd)) / r, float(np.sqrt(ee)) / r
 
     def refresh_reference(self):
-        """Self reference: u_ref <- u; the change terms of the state vanish exactly (problem.py:397-403)."""
+        """Self reference: u_ref <- u (problem.py:397-403). The proximity, JD and curl change terms of the
+        state are then exactly zero; image and barrier do not depend on the reference."""
         self.ref.copy_(self.u)
-        self.energy = self.evaluate_energy(self.u, self.warped)
+        if self.energy is None:
+            self.energy = self.evaluate_energy(self.u, self.warped)
+        self.energy = Energies(self.energy.image, 0., 0., 0., self.energy.barrier)
 
     def gradient_norm(self):
         (gg,) = self.dots((self.g, self.g))

@@ -65,6 +65,24 @@ class FusedQ1:
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
+    def soft_energy_f32(self, u, ref, base=None):
+        """FP32 screening [jd, curl, difference] (triton_q1.py:652-662), Kahan-compensated."""
+        self._check(u, ref) if base is None else self._check(u, ref, base)
+        out = torch.empty(3, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_SOFT)
+        _lib.call("seqvp_q1_soft_energy_f32", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
+                  *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def barrier_energy_f32(self, u, base=None):
+        """FP32 screening [value, min J, unweighted change, min base J] (triton_q1.py:663-678)."""
+        self._check(u) if base is None else self._check(u, base)
+        out = torch.empty(4, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_BARRIER)
+        _lib.call("seqvp_q1_barrier_energy_f32", _lib.ptr(u), _lib.ptr(base), _lib.ptr(self.k), *self._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
     def node_energy(self, warped, fixed, u, ref, lam, data_weight=None, base_warped=None, base_u=None):
         """[image, proximity, image difference vs base, proximity difference vs base]."""
         self._check(u, ref)

@@ -191,3 +191,4 @@ class CpuPairState:
     def refresh_reference(self):
         self.ref = self._t(self.u.copy())
         self.energy = self.evaluate_energy(self.u, self.warped)
+        assert (self.energy.proximity, self.energy.jd, self.energy.curl) == (0., 0., 0.)

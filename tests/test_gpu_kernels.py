@@ -300,3 +300,26 @@ def test_node_terms(gpu_device):
     np.testing.assert_allclose(tot.cpu().numpy(), want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
     np.testing.assert_allclose(img.cpu().numpy(), m.node_gradient_base(u, ref, warped, b, total=False), rtol=1e-14)
     np.testing.assert_allclose(ggn.cpu().numpy(), m.ggn_diagonal_base(b), rtol=1e-14)
+
+
+@pytest.mark.parametrize("activation", [.05, .9])
+def test_fp32_screen_energies_within_band(gpu_device, activation):
+    """The FP32 screening energies stay within the screen's band of the FP64 values
+    (64 ulps of binary32 for JD/curl, 4096 for the barrier; accelerated.py:67-76)."""
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case fp32 screen", activation, flush=True)
+    prob, og, m = case(activation)
+    ref, base, u = fields(og.shape, 130)
+    q = FusedQ1(prob, gpu_device)
+    U, R, B = dev(u, gpu_device), dev(ref, gpu_device), dev(base, gpu_device)
+    s64, s32 = q.soft_energy(U, R, B).cpu().numpy(), q.soft_energy_f32(U, R, B).cpu().numpy()
+    b64, b32 = q.barrier_energy(U, B).cpu().numpy(), q.barrier_energy_f32(U, B).cpu().numpy()
+    eps = 2. ** -24
+    assert abs(s32[0] - s64[0]) <= 64 * eps * s64[0] and abs(s32[1] - s64[1]) <= 64 * eps * s64[1]
+    assert abs(b32[0] - b64[0]) <= 4096 * eps * max(b64[0], 1e-300) + 1e-300
+    assert abs(b32[1] - b64[1]) <= 8 * eps * abs(b64[1])
+    jb, cb, _ = q.soft_energy(B, R).cpu().numpy()
+    bb = q.barrier_energy(B).cpu().numpy()[0]
+    band = eps * (64 * (s64[0] + s64[1] + jb + cb) + 4096 * (b64[0] + bb))
+    w = prob.barrier_weight
+    assert abs((s32[2] + w * b32[2]) - (s64[2] + w * b64[2])) <= band

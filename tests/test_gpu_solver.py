@@ -39,3 +39,33 @@ def test_device_solver_matches_cpu_twin_stepwise(gpu_device):
             assert abs(a[key] - b[key]) <= 1e-10 * abs(a[key]), key
     np.testing.assert_allclose(gpu.theta.cpu().numpy(), cpu.theta.numpy(), rtol=0,
                                atol=1e-10 * np.abs(cpu.theta.numpy()).max())
+
+
+def test_fp32_screen_keeps_the_fp64_decisions(gpu_device):
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print("case fp32 screen driver 20x22x24", flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    spec = PairSpec((20, 22, 24), 8, (2., 5.), 3, .6, .01, 5)
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    runs = []
+    for screen in (False, True):
+        st = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+        st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=gpu_device))
+        rows = []
+        res = solve_sequential(st, SequentialConfig(), PGConfig(fp32_screen=screen), max_outer=3, observer=rows.append)
+        runs.append((res, rows, st))
+    (r64, rows64, st64), (r32, rows32, st32) = runs
+    assert r64.status == r32.status and r64.outer_accepted == r32.outer_accepted
+    assert [(a["reason"], a["inner_accepted_steps"], a["inner_trials"]) for a in rows64] == \
+           [(a["reason"], a["inner_accepted_steps"], a["inner_trials"]) for a in rows32]
+    assert torch.equal(st64.theta, st32.theta)
+    rec = st32.screen_record
+    assert rec["uses"] > 0 and rec["checks"] >= 1 and not rec["disabled"] and rec["max_rel"] <= 1.

@@ -154,6 +154,14 @@ int seqvp_dst_poisson(const double* rhs, const double* den, const double* scale,
                       int64_t my, int64_t mz, int64_t ncomp, double* out, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* Spectral denominator omega*(a + b*lambda) of the Q1 (q1 = 1) or seven-point (q1 = 0) metric on the
+ * interior DST-I modes, built on the device from its separable 1-D factors lx/wx (length mx), ly/wy (my),
+ * lz/wz (mz) (replaces poisson_spectrum and the denominator of build_poisson_inverse, poisson.py:16-30,
+ * 118-131; the products are formed in the reference's order without contraction). den: (mz, my, mx). */
+int seqvp_poisson_denominator(const double* lx, const double* wx, const double* ly, const double* wy,
+                              const double* lz, const double* wz, int64_t mx, int64_t my, int64_t mz, int q1,
+                              double omega, double a, double b, double* den, void* stream);
+
 /* FP32 variant for the precision form (replaces TorchPoisson.apply_fp32, torch_ops.py:117-136):
  * rhs/out float64 in memory, transforms and division in float32 on MFMA. */
 int seqvp_dst_poisson_f32(const double* rhs, const double* den, const double* scale, int negate, int64_t mx,

@@ -37,6 +37,8 @@ SIGNATURES = {
                             c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp],
     "seqvp_dst_poisson": [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_dst_poisson_f32": [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_poisson_denominator": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_int, c_dbl, c_dbl,
+                                  c_dbl, c_vp, c_vp],
     "seqvp_dots": [c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_inject": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],

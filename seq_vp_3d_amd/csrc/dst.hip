@@ -214,6 +214,29 @@ __global__ void mfma_probe_kernel(const double* __restrict__ a, const double* __
   for (int r = 0; r < 4; ++r) d[acc_row<T>(lk, r) * 16 + li] = (double)acc[r];
 }
 
+// Spectral denominator omega * (a + b * lambda) on the interior modes from its separable 1-D factors
+// (poisson.py:16-30, 118-131): q1 metric lambda = (lx my) mz + (mx ly) mz + (mx my) lz, seven-point
+// lambda = (lx + ly) + lz, products and sums in the reference's order.
+__global__ void __launch_bounds__(256) poisson_denominator_kernel(const double* __restrict__ lx,
+                                                                  const double* __restrict__ wx,
+                                                                  const double* __restrict__ ly,
+                                                                  const double* __restrict__ wy,
+                                                                  const double* __restrict__ lz,
+                                                                  const double* __restrict__ wz, int64_t mx,
+                                                                  int64_t my, int64_t mz, int q1, double omega,
+                                                                  double a, double b, double* __restrict__ den) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= mx * my * mz) return;
+  const int64_t x = i % mx, y = (i / mx) % my, z = i / (mx * my);
+  double lam;
+  if (q1)
+    lam = (lx[x] * wy[y] * wz[z] + wx[x] * ly[y] * wz[z]) + wx[x] * wy[y] * lz[z];
+  else
+    lam = (lx[x] + ly[y]) + lz[z];
+  den[i] = omega * (a + b * lam);
+}
+
 }  // namespace seqvp
 
 using namespace seqvp;
@@ -288,4 +311,15 @@ extern "C" int seqvp_dst_poisson_f32(const double* rhs, const double* den, const
                                      int64_t my, int64_t mz, int64_t ncomp, double* out, void* ws, size_t ws_bytes,
                                      void* stream) {
   return dst_poisson<float>(rhs, den, scale, negate, mx, my, mz, ncomp, out, ws, ws_bytes, stream);
+}
+
+extern "C" int seqvp_poisson_denominator(const double* lx, const double* wx, const double* ly, const double* wy,
+                                         const double* lz, const double* wz, int64_t mx, int64_t my, int64_t mz,
+                                         int q1, double omega, double a, double b, double* den, void* stream) {
+  if (!lx || !ly || !lz || !den || mx < 1 || my < 1 || mz < 1) return SEQVP_EINVAL;
+  if (q1 && (!wx || !wy || !wz)) return SEQVP_EINVAL;
+  const int64_t n = mx * my * mz;
+  hipLaunchKernelGGL(poisson_denominator_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, lx, wx, ly,
+                     wy, lz, wz, mx, my, mz, q1, omega, a, b, den);
+  return launch_status();
 }

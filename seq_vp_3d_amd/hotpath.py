@@ -23,8 +23,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .poisson import DevicePoisson, build_poisson_inverse
-from .prescreen import prescreen_flags
+from .poisson import DevicePoisson, SpectralMetric
+from .prescreen import device_table, prescreen_flags
 from .problem import q1_coefficients
 from .q1 import FusedJDRoot, FusedQ1
 from .spline import DeviceSpline, build_image_sampler
@@ -106,8 +106,11 @@ class PairState:
         self.dot_out = z(4)
         self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
         D, _ = q1_coefficients(g.spacing, "closed27_uniform")
-        self.D = D
+        self.D = device_table(D, self.device)
+        if self.D is None:
+            self.D = D        # unusable table: prescreen_flags proves nothing (every cell left undecided)
         self.poisson = None
+        self._den = None
         self.energy = None          # Energies of the accepted state
         self.metric_info = {}
 
@@ -170,8 +173,12 @@ class PairState:
         alpha = shift_factor * (bb / (self.nfree // 3)) / 3
         a = w.lam + alpha
         bp = w.mu / 2 + w.nu / 2
-        inverse = build_poisson_inverse(self.grid, a=a, b=bp, metric="q1")
-        self.poisson = DevicePoisson(inverse, self.device, ws=q.ws)
+        if not (np.isfinite(a) and a >= 0):
+            raise FloatingPointError("shifted metric not representable")
+        if self._den is None:
+            mz, my, mx = self.grid.interior_shape
+            self._den = torch.empty((mz, my, mx), dtype=torch.float64, device=self.device)
+        self.poisson = DevicePoisson(SpectralMetric(self.grid, a, bp, "q1"), self.device, ws=q.ws, out=self._den)
         inv_h2 = sum(1 / h ** 2 for h in self.grid.spacing)
         m = self.grid.omega * (a + bp * (8 / 9) * inv_h2)
         node = q.node_min_jd(self.u) if jd_affine else None

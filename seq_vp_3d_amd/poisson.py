@@ -1,8 +1,9 @@
 """Device Q1 Poisson metric inverse via DST-I (the TorchPoisson / _HostPoisson interface).
 
-poisson_spectrum / build_poisson_inverse follow jd_curl/three_d/poisson.py:16-30, 118-131 (the
-denominator is built once per metric on the host, in float64, exactly as the reference does and
-uploaded); apply / apply_fp32 / apply_equilibrated follow torch_ops.py:98-136 and
+poisson_spectrum / build_poisson_inverse follow jd_curl/three_d/poisson.py:16-30, 118-131 on the host
+(for callers that hand over a PoissonInverse3D); the solver builds the same denominator on the device
+from the separable 1-D spectrum factors (SpectralMetric), bitwise equal and without the 53 MB host
+array per metric build. apply / apply_fp32 / apply_equilibrated follow torch_ops.py:98-136 and
 hybrid.py:230-286, with every transform running in libseqvp_hip.so (seqvp_dst_poisson[_f32]).
 """
 from dataclasses import dataclass
@@ -14,6 +15,17 @@ from . import _lib
 from .problem import InputValidationError
 
 OP_DST = 6
+
+
+def spectrum_factors(grid):
+    """The 1-D factors of the spectrum per axis x, y, z: ell = (2 sin(theta/2)/h)^2 and m = (2 + cos theta)/3,
+    theta = pi k/(n-1), k = 1..n-2 (poisson.py:19-23)."""
+    ell, mass = [], []
+    for n, h in zip(grid.shape[::-1], grid.spacing):
+        theta = np.pi * np.arange(1, n - 1) / (n - 1)
+        ell.append((2 * np.sin(theta / 2) / h) ** 2)
+        mass.append((2 + np.cos(theta)) / 3)
+    return ell, mass
 
 
 def poisson_spectrum(grid, metric="q1"):
@@ -52,15 +64,39 @@ def build_poisson_inverse(grid, *, a, b, metric="q1"):
     return PoissonInverse3D(grid, float(a), float(b), metric, den)
 
 
-class DevicePoisson:
-    """M^-1 on the device for one metric; same methods as TorchPoisson (torch_ops.py:107-136)."""
+@dataclass(frozen=True)
+class SpectralMetric:
+    """A Poisson metric given by (grid, a, b, metric) whose denominator is built on the device."""
+    grid: object
+    a: float
+    b: float
+    metric: str = "q1"
 
-    def __init__(self, inverse, device="cuda", ws=None):
+
+class DevicePoisson:
+    """M^-1 on the device for one metric; same methods as TorchPoisson (torch_ops.py:107-136).
+    `inverse` is a PoissonInverse3D (denominator uploaded) or a SpectralMetric (denominator built on the
+    device from the separable 1-D spectrum factors, no host array)."""
+
+    def __init__(self, inverse, device="cuda", ws=None, out=None):
         self.grid = inverse.grid
         self.device = torch.device(device)
         self.inverse = inverse
-        self._denominator = torch.tensor(np.ascontiguousarray(inverse.denominator), dtype=torch.float64,
-                                         device=self.device)
+        if isinstance(inverse, SpectralMetric):
+            if not (np.isfinite(inverse.a) and inverse.a >= 0 and np.isfinite(inverse.b) and inverse.b > 0):
+                raise InputValidationError("a >= 0 and b > 0 required")
+            ell, mass = spectrum_factors(self.grid)
+            fac = [torch.tensor(v, dtype=torch.float64, device=self.device) for v in ell + mass]
+            mz, my, mx = self.grid.interior_shape
+            den = out if out is not None else torch.empty((mz, my, mx), dtype=torch.float64, device=self.device)
+            _lib.call("seqvp_poisson_denominator", _lib.ptr(fac[0]), _lib.ptr(fac[3]), _lib.ptr(fac[1]),
+                      _lib.ptr(fac[4]), _lib.ptr(fac[2]), _lib.ptr(fac[5]), mx, my, mz, int(inverse.metric == "q1"),
+                      self.grid.omega, float(inverse.a), float(inverse.b), _lib.ptr(den), _lib.stream_handle(self.device))
+            self._factors = fac      # kept alive until the launch has run
+            self._denominator = den
+        else:
+            self._denominator = torch.tensor(np.ascontiguousarray(inverse.denominator), dtype=torch.float64,
+                                             device=self.device)
         self.mz, self.my, self.mx = self.grid.interior_shape
         self.n = 3 * self.mz * self.my * self.mx
         self.ws = ws if ws is not None else _lib.Workspace(self.device)

@@ -13,6 +13,8 @@ from .problem import InputValidationError
 
 
 def _table(coefficients, device):
+    if isinstance(coefficients, torch.Tensor):          # a device table already checked by device_table()
+        return coefficients, True
     D = np.ascontiguousarray(np.asarray(coefficients, dtype=np.float64))
     if D.shape != (3, 27, 8):
         raise InputValidationError("prescreen: closed27 coefficient table (3, 27, 8) required")
@@ -57,3 +59,9 @@ def prescreen_flags(u, grid, coefficients, delta, flags=None, count=None):
     _lib.call("seqvp_prescreen_flags", _lib.ptr(uc), _lib.ptr(D), float(delta), nx, ny, nz,
               _lib.ptr(flags), _lib.ptr(count), _lib.stream_handle(u.device))
     return flags, count
+
+
+def device_table(coefficients, device):
+    """The closed27 table uploaded once (None when it has a subnormal or nonfinite entry: no proofs)."""
+    D, usable = _table(coefficients, device)
+    return D if usable else None

@@ -323,3 +323,13 @@ def test_fp32_screen_energies_within_band(gpu_device, activation):
     band = eps * (64 * (s64[0] + s64[1] + jb + cb) + 4096 * (b64[0] + bb))
     w = prob.barrier_weight
     assert abs((s32[2] + w * b32[2]) - (s64[2] + w * b64[2])) <= band
+
+
+@pytest.mark.parametrize("metric", ["q1", "seven_point"])
+def test_device_denominator_matches_host(gpu_device, metric):
+    from seq_vp_3d_amd.poisson import DevicePoisson, SpectralMetric, build_poisson_inverse
+    from seq_vp_3d_amd.problem import make_grid_3d
+    grid = make_grid_3d(nx=23, ny=19, nz=17, hx=1., hy=1.3, hz=.8)
+    host = build_poisson_inverse(grid, a=.21, b=.037, metric=metric).denominator
+    devp = DevicePoisson(SpectralMetric(grid, .21, .037, metric), gpu_device)
+    np.testing.assert_allclose(devp._denominator.cpu().numpy(), host, rtol=2e-16, atol=0)

@@ -4,7 +4,7 @@ Workload (synthetic pair, built as the 160x192x224 native configuration: 200 Gau
 3-12 voxels, 6 sinusoid modes per component, true JD >= 0.5, noise 0.01): one outer step =
 the metric build (GGN diagonal, image shift, T62 node JD, equilibration scale) followed by
 `--inner` projected-gradient iterations, each with the full gradient, the equilibrated DST-I
-direction, the T66 JD-root step bound, one trial (candidate map, Bernstein prescreen of every cell,
+direction, the T66 JD-root step bound, one trial (candidate map, whole-cell JD certificate of every cell,
 spline sampling, energies and stable differences against the accepted state) and the promotion of
 an accepted trial. Inputs are resident in HBM before the timed region; only scalars cross PCIe.
 Prints one JSON line.
@@ -27,7 +27,6 @@ FLOPS_PER_UNIT = {
     "seqvp_q1_gradient": 12971. + 8093.,
     "seqvp_q1_ggn_diagonal": 27290.,
     "seqvp_jd_root": 4015.,
-    "seqvp_prescreen_flags": 6500.,
     "seqvp_spline_warp": 1078.,
 }
 FP64_PEAK_TFLOPS = 78.6

@@ -217,6 +217,105 @@ int seqvp_metric_scale(const double* diag, const double* node, double m, double 
  * assume; a layout self-check. */
 int seqvp_mfma_probe(const double* a, const double* b, int fp32, double* d, void* stream);
 
+/* ---- whole-cell JD certificate (replaces certify_map with PrescreenCertificateConfig,
+ *      jd_curl/three_d/feasibility.py:395-769; compiled_enclosure.py; compiled_fallback.py) ---------------- */
+
+/* Certificate reasons (FeasibilityResult.reason strings). */
+#define SEQVP_CERT_POSITIVE_ENCLOSURE_COVER 0 /* certified; some cells proven by the hull or the prescreen */
+#define SEQVP_CERT_POSITIVE_BERNSTEIN_COVER 1 /* certified by the Bernstein fallback alone */
+#define SEQVP_CERT_JACOBIAN_VIOLATION 2       /* a cell holds a point with J <= delta */
+#define SEQVP_CERT_ARITHMETIC_UNCERTAIN 3     /* an interval step was nonfinite or inverted */
+#define SEQVP_CERT_SUBDIVISION_BUDGET 4       /* a cell exhausted its box or depth budget */
+#define SEQVP_CERT_NONFINITE_STATE 5
+#define SEQVP_CERT_NONZERO_BOUNDARY 6
+#define SEQVP_CERT_INVALID_DOMAIN 7
+
+/* Map certificate (the FeasibilityResult/PrescreenFeasibilityResult fields, feasibility.py:136-173).
+ * status: 0 certified_feasible, 1 uncertain, 2 infeasible. NaN stands for None. */
+typedef struct {
+  int32_t status;
+  int32_t reason;             /* SEQVP_CERT_* */
+  int32_t max_depth;
+  int32_t arithmetic_failure;
+  int64_t n_cells_checked;
+  int64_t n_uncertain;
+  int64_t n_infeasible;
+  int64_t n_boxes;
+  int64_t n_fast_cells;       /* proven by the edge hull */
+  int64_t n_prescreen_cells;  /* proven by the T75 prescreen */
+  int64_t n_fallback_cells;   /* classified by the interval Bernstein fallback */
+  int64_t witness_cell;       /* first infeasible cell in tile order, -1 when none */
+  double witness_point[3];    /* its reference point (x, y, z) in [0,1]^3 */
+  double min_margin;          /* min_margin_lower_bound: lower bound of min (J - delta) */
+  double min_jd;              /* min_jd_lower_bound */
+  double sampled_min_jd;      /* float64 sample minimum over fallback cells (NaN when any cell was proven early) */
+} SeqvpCertificate;
+
+/* One cell's Bernstein classification (CellCertificate, feasibility.py:124-133).
+ * reason: 0 positive_bernstein_cover, 1 sample_witness, 2 subdivision_witness, 3 box_budget, 4 depth_budget,
+ * 5 arithmetic_uncertain. */
+typedef struct {
+  int32_t status;
+  int32_t reason;
+  int32_t max_depth;
+  int32_t has_witness;
+  int64_t n_boxes;
+  double witness_point[3];
+  double min_margin;
+  double min_jd;
+  double sampled_min_jd;
+} SeqvpCellCertificate;
+
+/* Device workspace of seqvp_certificate_screen (tile = cells per certificate tile, 8192 in the reference). */
+size_t seqvp_certificate_workspace_bytes(int64_t nx, int64_t ny, int64_t nz, int64_t tile);
+
+/* Edge-hull margins on the device, bitwise the CPU kernel's (replaces parallel_margins,
+ * compiled_enclosure.py:193-227, 336-345, and the GPU edge hull of backends/edge_hull_gpu.py):
+ * margin[c] = lo(J_hull - delta), flag[c] = 1 when an interval step was nonfinite or inverted (flag may be NULL). */
+int seqvp_edge_hull(const double* u, double hx, double hy, double hz, double delta, int64_t nx, int64_t ny,
+                    int64_t nz, double* margin, uint8_t* flag, void* stream);
+
+/* Device screen of a candidate map (the hull and prescreen stages of _certify_map_parallel,
+ * feasibility.py:571-695): edge hull with the whole-tile rule, then the T75 prescreen (D = closed27 derivative
+ * table (3,27,8) on the device, NULL = no prescreen) of the cells the hull left. summary (5 device doubles) =
+ * [min proven hull margin, min proven prescreen margin (+inf: none), n_fast, n_prescreen, n_undecided];
+ * undecided[0 : min(n_undecided, capacity)] receives the undecided cell ids in no particular order;
+ * codes (optional, ncell bytes) = 0 hull, 1 prescreen, 2 undecided. */
+int seqvp_certificate_screen(const double* u, const double* D, double hx, double hy, double hz, double delta,
+                             int64_t nx, int64_t ny, int64_t nz, int64_t tile, uint8_t* codes, int64_t* undecided,
+                             int64_t capacity, double* summary, void* workspace, size_t workspace_bytes, void* stream);
+
+/* out (n, 8, 3) = the vertex displacements of cells[0:n] (vertex a = x + 2y + 4z), on the device
+ * (the flat[_node_indices_for(grid, cells)] gather of feasibility.py:712-713). */
+int seqvp_gather_vertices(const double* u, const int64_t* cells, int64_t n, int64_t nx, int64_t ny, int64_t nz,
+                          double* out, void* stream);
+
+/* HOST. The interval arrays of the Bernstein fallback for n cells of host vertices (n, 8, 3): p = J - delta at
+ * the 27 points, its Bernstein coefficients and the float64 sample J, each (n, 27) (replaces
+ * compiled_fallback.fallback_arrays / feasibility._fallback_arrays, feasibility.py:515-523).
+ * Returns 1 when every step of every cell was finite and ordered, 0 otherwise, SEQVP_EINVAL on bad arguments. */
+int seqvp_bernstein_arrays(const double* vertices, int64_t n, const double* spacing, double delta, double* p_lo,
+                           double* p_hi, double* c_lo, double* c_hi, double* samples);
+
+/* HOST. Bernstein classification of one cell (jacobian_intervals + lagrange_to_bernstein + _classify with the
+ * lowest-bound-first subdivision, feasibility.py:305-393). spacing = (hx, hy, hz). */
+int seqvp_bernstein_cell(const double* vertices, const double* spacing, double delta, int max_depth, int64_t max_boxes,
+                         SeqvpCellCertificate* out);
+
+/* HOST. The fallback and final record of a screened map (feasibility.py:697-769): screen = the 5 summary values
+ * of seqvp_certificate_screen (host copy), ncell = cells of the map, vertices (n, 8, 3) / cells (n, strictly
+ * increasing) = the undecided cells. Jobs (one tile's cells each) run on `workers` threads. */
+int seqvp_certificate_finish(const double* screen, int64_t ncell, const double* vertices, const int64_t* cells,
+                             int64_t n, const double* spacing, double delta, int64_t tile, int max_depth,
+                             int64_t max_boxes, int workers, SeqvpCertificate* out);
+
+/* HOST. The whole certificate of a host map u (nz, ny, nx, 3) on the CPU (certify_map, feasibility.py:395-431 and
+ * _certify_map_parallel): nonfinite/boundary/domain checks against the axes ax/ay/az, edge hull, prescreen (D =
+ * host derivative table, NULL = none), Bernstein fallback. The independent re-certification of an output. */
+int seqvp_certify_map_host(const double* u, int64_t nx, int64_t ny, int64_t nz, const double* ax, const double* ay,
+                           const double* az, const double* spacing, double delta, const double* D, int64_t tile,
+                           int max_depth, int64_t max_boxes, int workers, SeqvpCertificate* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,8 +49,35 @@ SIGNATURES = {
     "seqvp_max_abs_diff": [c_vp, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_metric_scale": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_mfma_probe": [c_vp, c_vp, c_int, c_vp, c_vp],
+    "seqvp_certificate_workspace_bytes": [c_i64, c_i64, c_i64, c_i64],
+    "seqvp_edge_hull": [c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_certificate_screen": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp,
+                                 c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_gather_vertices": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],
+    "seqvp_bernstein_arrays": [c_vp, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "seqvp_bernstein_cell": [c_vp, c_vp, c_dbl, c_int, c_i64, c_vp],
+    "seqvp_certificate_finish": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_i64, c_int, c_i64, c_int, c_vp],
+    "seqvp_certify_map_host": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_i64, c_int, c_i64,
+                               c_int, c_vp],
 }
-_RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz}
+_RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz,
+             "seqvp_certificate_workspace_bytes": c_sz}
+
+
+class Certificate(ctypes.Structure):
+    """SeqvpCertificate (include/seqvp.h)."""
+    _fields_ = [("status", ctypes.c_int32), ("reason", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("arithmetic_failure", ctypes.c_int32), ("n_cells_checked", c_i64), ("n_uncertain", c_i64),
+                ("n_infeasible", c_i64), ("n_boxes", c_i64), ("n_fast_cells", c_i64), ("n_prescreen_cells", c_i64),
+                ("n_fallback_cells", c_i64), ("witness_cell", c_i64), ("witness_point", c_dbl * 3),
+                ("min_margin", c_dbl), ("min_jd", c_dbl), ("sampled_min_jd", c_dbl)]
+
+
+class CellCertificate(ctypes.Structure):
+    """SeqvpCellCertificate (include/seqvp.h)."""
+    _fields_ = [("status", ctypes.c_int32), ("reason", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("has_witness", ctypes.c_int32), ("n_boxes", c_i64), ("witness_point", c_dbl * 3),
+                ("min_margin", c_dbl), ("min_jd", c_dbl), ("sampled_min_jd", c_dbl)]
 
 ERRORS = {-1: "invalid argument", -2: "kernel launch failed", -3: "workspace too small"}
 
@@ -107,6 +134,20 @@ def call(name, *args):
         timer.events.setdefault(name, []).append((start, end))
     if code != 0:
         raise SeqvpError(f"{name}: {ERRORS.get(code, code)}")
+
+
+def raw(name, *args):
+    """The C-ABI call's integer result, unchecked (for entry points whose result is a value)."""
+    return getattr(load(), name)(*args)
+
+
+def hptr(a):
+    """Host address of a C-contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    if not a.flags.c_contiguous:
+        raise ValueError("C-contiguous host array expected")
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def ptr(t):

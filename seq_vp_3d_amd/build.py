@@ -1,7 +1,8 @@
 """Builds libseqvp_hip.so (gfx950) in-tree with hipcc.
 
-Every translation unit is compiled for gfx950 only. The prescreen unit is compiled without
-floating-point contraction, because its proof bound assumes every +, -, * is rounded separately.
+Every translation unit is compiled for gfx950 only. The prescreen and certificate units are compiled
+without floating-point contraction: the prescreen's proof bound assumes every +, -, * is rounded
+separately, and the interval certificate widens each separately rounded result by one ulp.
 Packed FP32 instructions are disabled for all units.
 """
 import os
@@ -15,10 +16,10 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libseqvp_hip.so"
 OBJ = HERE / "_build"
 
-SOURCES = ("q1_sweeps.hip", "prescreen.hip", "spline.hip", "dst.hip", "vec.hip", "abi.hip")
+SOURCES = ("q1_sweeps.hip", "prescreen.hip", "certificate.hip", "spline.hip", "dst.hip", "vec.hip", "abi.hip")
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-PER_FILE = {"prescreen.hip": ["-ffp-contract=off"]}
+PER_FILE = {"prescreen.hip": ["-ffp-contract=off"], "certificate.hip": ["-ffp-contract=off"]}
 
 
 def hipcc():

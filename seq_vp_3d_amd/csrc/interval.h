@@ -1,0 +1,115 @@
+// Binary64 interval arithmetic with one outward step per elementary operation, shared by the GPU
+// certificate screen and the CPU certificate (host and device compile the same code).
+//
+// Semantics follow jd_curl/three_d/interval.py:31-124 and the per-cell kernels of
+// compiled_enclosure.py:143-227 / compiled_fallback.py:62-160: every +, -, *, / of two intervals takes the
+// rounded-to-nearest endpoint results (for * and / the min and max of the four endpoint products) and
+// widens them by one ulp outward (nextafter towards -inf / +inf); an operation is valid when both widened
+// endpoints are finite and ordered. Units including this header are compiled without floating-point
+// contraction, so each operation is the separately rounded IEEE result the reference computes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#pragma clang fp contract(off)
+
+namespace seqvp {
+namespace iv {
+
+__host__ __device__ __forceinline__ uint64_t bits_of(double x) {
+  uint64_t b;
+  memcpy(&b, &x, 8);
+  return b;
+}
+__host__ __device__ __forceinline__ double from_bits(uint64_t b) {
+  double x;
+  memcpy(&x, &b, 8);
+  return x;
+}
+__host__ __device__ __forceinline__ bool is_nan(double x) { return !(x == x); }
+__host__ __device__ __forceinline__ bool is_finite(double x) { return x - x == 0.0; }
+
+// nextafter(x, +inf) on the bits: the successor of x in the ordered binary64 line; +0 and -0 both step to
+// the smallest positive subnormal, +inf and NaN are fixed points.
+__host__ __device__ __forceinline__ double next_up(double x) {
+  if (is_nan(x) || x == __builtin_inf()) return x;
+  if (x == 0.0) return from_bits(1ull);
+  const uint64_t b = bits_of(x);
+  return from_bits(x > 0.0 ? b + 1 : b - 1);
+}
+// nextafter(x, -inf).
+__host__ __device__ __forceinline__ double next_down(double x) { return -next_up(-x); }
+
+// NumPy minimum/maximum: NaN propagates; otherwise the comparison result (ties return the second operand).
+__host__ __device__ __forceinline__ double np_min(double a, double b) {
+  if (is_nan(a)) return a;
+  if (is_nan(b)) return b;
+  return a < b ? a : b;
+}
+__host__ __device__ __forceinline__ double np_max(double a, double b) {
+  if (is_nan(a)) return a;
+  if (is_nan(b)) return b;
+  return a > b ? a : b;
+}
+
+struct I {
+  double lo, hi;
+};
+
+// outward(lo, hi) and its validity (finite, ordered).
+__host__ __device__ __forceinline__ I outward(double lo, double hi, bool& ok) {
+  const I r{next_down(lo), next_up(hi)};
+  ok = ok && is_finite(r.lo) && is_finite(r.hi) && r.lo <= r.hi;
+  return r;
+}
+__host__ __device__ __forceinline__ I add(I a, I b, bool& ok) { return outward(a.lo + b.lo, a.hi + b.hi, ok); }
+__host__ __device__ __forceinline__ I sub(I a, I b, bool& ok) { return outward(a.lo - b.hi, a.hi - b.lo, ok); }
+__host__ __device__ __forceinline__ I mul(I a, I b, bool& ok) {
+  const double x = a.lo * b.lo, y = a.lo * b.hi, z = a.hi * b.lo, w = a.hi * b.hi;
+  return outward(np_min(np_min(np_min(x, y), z), w), np_max(np_max(np_max(x, y), z), w), ok);
+}
+__host__ __device__ __forceinline__ I point(double v) { return I{v, v}; }
+
+// det of the interval matrix F[r][j] by the left-associated cofactor expansion along row 0
+// (compiled_enclosure.py:60-71; feasibility.py:213-216).
+__host__ __device__ __forceinline__ I determinant(const I F[3][3], bool& ok) {
+  const I a = sub(mul(F[1][1], F[2][2], ok), mul(F[1][2], F[2][1], ok), ok);
+  const I b = sub(mul(F[1][2], F[2][0], ok), mul(F[1][0], F[2][2], ok), ok);
+  const I c = sub(mul(F[1][0], F[2][1], ok), mul(F[1][1], F[2][0], ok), ok);
+  return add(add(mul(F[0][0], a, ok), mul(F[0][1], b, ok), ok), mul(F[0][2], c, ok), ok);
+}
+
+// Edge-hull margin of one cell (compiled_enclosure.py:193-227): F[r][j] = hull of the four raw edge
+// differences along j (widened once, then divided by h_j with one more outward step) plus the identity,
+// margin = lo(det F - delta). `ok` is false when any step was nonfinite or inverted (the cell's flag).
+// Vertex a = x + 2y + 4z; uv[a][r] is component r of the displacement at vertex a.
+__host__ __device__ inline double edge_hull_margin(const double uv[8][3], const double h[3], double delta,
+                                                   bool& ok) {
+  I F[3][3];
+  for (int j = 0; j < 3; ++j) {
+    for (int r = 0; r < 3; ++r) {
+      double lower = __builtin_inf(), upper = -__builtin_inf();
+      bool good = true;
+      for (int a = 0; a < 8; ++a) {
+        if (a & (1 << j)) continue;
+        const int b = a | (1 << j);
+        const double d = uv[b][r] - uv[a][r];
+        if (!is_finite(d)) good = false;
+        lower = np_min(lower, d);
+        upper = np_max(upper, d);
+      }
+      I e = outward(lower, upper, good);
+      e = outward(e.lo / h[j], e.hi / h[j], good);
+      const double id = r == j ? 1.0 : 0.0;
+      F[r][j] = add(e, point(id), good);
+      ok = ok && good;
+    }
+  }
+  const I J = determinant(F, ok);
+  const I m = outward(J.lo - delta, J.hi - delta, ok);
+  return m.lo;
+}
+
+}  // namespace iv
+}  // namespace seqvp

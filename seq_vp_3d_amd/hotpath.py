@@ -12,8 +12,9 @@ Operations and the reference code they follow:
   gradient()   -- TorchLinearization.gradient (torch_ops.py:559-589) restricted to theta
   direction()  -- p = -S M^-1 S g (poisson_gradient.py:1241-1262, hybrid.py:262-286)
   jd_bound()   -- T66 global root on the direction (torch_ops.py:636-673)
-  trial()      -- evaluate_trial's device part: candidate map, prescreen, sampling, energies vs base
-                  (accelerated.py:1666-1730, 1574-1664)
+  trial()      -- evaluate_trial: candidate map, whole-cell JD certificate (GPU edge hull + prescreen of every
+                  cell, CPU interval Bernstein fallback of the undecided ones), sampling, energies vs base
+                  (accelerated.py:1666-1730, 1574-1664; feasibility.py:571-769)
   promote()    -- promote_trial: the gradient sampling of an accepted trial (accelerated.py:2850-2874)
 """
 import ctypes
@@ -24,8 +25,7 @@ import torch
 
 from . import _lib
 from .poisson import DevicePoisson, SpectralMetric
-from .prescreen import device_table, prescreen_flags
-from .problem import q1_coefficients
+from .certificate import CertificateConfig, DeviceCertificate
 from .q1 import FusedJDRoot, FusedQ1
 from .spline import DeviceSpline, build_image_sampler
 
@@ -63,14 +63,15 @@ class TrialResult:
     total_difference: float = float("nan")     # E(trial) - E(base), stable
     outer_difference: float = float("nan")     # F(trial) - F(base), stable
     min_gap: float = float("nan")
-    unproven_cells: int = 0
+    unproven_cells: int = 0                    # n_uncertain + n_infeasible of a failed certificate
     band: float | None = None                  # FP32 screening band of the differences (None: FP64 values)
+    certificate: object = None                 # FeasibilityResult of the candidate (None: not reached)
 
 
 class PairState:
     """Device buffers of one pair on one grid (node fields (nz, ny, nx, 3), free vectors (3F,))."""
 
-    def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None):
+    def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None, certificate=None):
         self.problem = problem
         self.grid = problem.grid
         self.device = torch.device(device)
@@ -100,15 +101,14 @@ class PairState:
         self.v = z(*self.shape, 3)
         self.scale = z(self.nfree)
         self.diag = z(self.nfree)
-        self.flags = torch.empty(g.n_cells, dtype=torch.uint8, device=self.device)
-        self.count = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.bad = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.dot_out = z(4)
         self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
-        D, _ = q1_coefficients(g.spacing, "closed27_uniform")
-        self.D = device_table(D, self.device)
-        if self.D is None:
-            self.D = D        # unusable table: prescreen_flags proves nothing (every cell left undecided)
+        config = certificate if certificate is not None else CertificateConfig(delta=problem.delta)
+        if config.delta != problem.delta:
+            raise ValueError("certificate delta must equal the problem's delta")
+        self.cert = DeviceCertificate(g, self.device, config)
+        self._certificate = None    # certificate of the current candidate (kept across its energy evaluations)
         self.poisson = None
         self._den = None
         self.energy = None          # Energies of the accepted state
@@ -209,14 +209,15 @@ class PairState:
 
     # ---- trial and promotion -----------------------------------------------------------------------
     def trial(self, tau, screen=False):
-        """Candidate theta + tau p: device prescreen, sampling and energies against the accepted state.
+        """Candidate theta + tau p: certificate screen, sampling and energies against the accepted state.
         screen=True evaluates the soft and barrier terms with the FP32 screening kernels and returns the
         screening band of the differences (accelerated.py:1603-1612, 67-76)."""
         nz, ny, nx = self.shape
         q = self.q1
         _lib.call("seqvp_axpy_inject", _lib.ptr(self.theta), _lib.ptr(self.p), float(tau), nx, ny, nz,
                   _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
-        prescreen_flags(self.u_t, self.grid, self.D, q.delta, self.flags, self.count)
+        self.cert.screen(self.u_t)
+        self._certificate = None
         _, _, bad = self.spline.warp(self.u_t, with_gradient=False, value=self.warped_t)
         self._trial_bad = bad
         return self.trial_energies(screen)
@@ -232,12 +233,18 @@ class PairState:
         else:
             soft = q.soft_energy(self.u_t, self.ref, self.u)
             bar = q.barrier_energy(self.u_t, self.u)
-        vals = torch.cat((node, soft, bar, self._trial_bad.to(torch.float64), self.count.to(torch.float64))).tolist()
-        n, s, b, nbad, unproven = vals[0:4], vals[4:7], vals[7:11], vals[11], int(vals[12])
+        vals = torch.cat((node, soft, bar, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
+        n, s, b, nbad = vals[0:4], vals[4:7], vals[7:11], vals[11]
         if nbad:
-            return TrialResult(False, "INVALID_DOMAIN", None, unproven_cells=unproven)
+            return TrialResult(False, "INVALID_DOMAIN", None)
         if not b[1] > q.delta or not np.isfinite(b[1]):
-            return TrialResult(False, "JACOBIAN_VIOLATION", None, unproven_cells=unproven)
+            return TrialResult(False, "JACOBIAN_VIOLATION", None)
+        cert = self._certificate if self._certificate is not None else self.cert.finish(summary=vals[12:17])
+        self._certificate = cert
+        if not cert.certified:
+            reason = "FEASIBILITY_UNCERTAIN" if cert.status == "uncertain" else cert.reason
+            return TrialResult(False, reason, None, unproven_cells=cert.n_uncertain + cert.n_infeasible,
+                               certificate=cert)
         energy = Energies(n[0], n[1], s[0], s[1], b[0])
         change = q.barrier_weight * b[2]
         total = n[2] + n[3] + s[2] + change
@@ -249,7 +256,7 @@ class PairState:
             soft_sum = max(energy.jd + energy.curl + base.jd + base.curl, 0.)
             bar_sum = max(energy.barrier + base.barrier, 0.)
             band = FP32_EPS * (FP32_BAND_SOFT * soft_sum + FP32_BAND_BARRIER * bar_sum)
-        return TrialResult(True, "", energy, total, outer, gap, unproven, band)
+        return TrialResult(True, "", energy, total, outer, gap, 0, band, cert)
 
     def promote(self, energy):
         """Make the trial the accepted state (swap buffers) and sample the image gradient there."""

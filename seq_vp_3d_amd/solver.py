@@ -5,10 +5,10 @@ ConservativePGConfig3D(resolution_exit=True, jd_step_global=True) and the
 image_shift_ggn_equilibrated_jd_affine metric (jd_curl/three_d/solvers/poisson_gradient.py:1090-1646), and
 solve_sequential (jd_curl/three_d/sequential.py:151-689) -- on a PairState whose arrays never leave HBM.
 Only scalars cross PCIe. The step bound is the T66 global closed27 root (no historical witness memory:
-its prediction is the proposal itself). A trial is valid only when every cell is proven J > delta by the
-GPU Bernstein prescreen and the closed27 samples are above delta; cells the prescreen cannot prove make
-the trial FEASIBILITY_UNCERTAIN (the authoritative CPU interval certificate for such cells is not part of
-this module).
+its prediction is the proposal itself). A trial is valid only when its closed27 samples are above delta and
+the whole-cell certificate proves J > delta on every cell (GPU edge hull and prescreen of every cell, the
+CPU interval Bernstein fallback for the cells they leave undecided; an uncertain certificate makes the trial
+FEASIBILITY_UNCERTAIN, a witness JACOBIAN_VIOLATION, as accelerated.py:1722-1729).
 """
 from dataclasses import dataclass, field
 from time import perf_counter
@@ -113,6 +113,18 @@ class SequentialResult:
     history: list
     seconds: float
     energy: object = None
+
+
+CERTIFICATE_FIELDS = ("status", "reason", "min_jd_lower_bound", "n_fast_cells", "n_prescreen_cells",
+                      "n_fallback_cells", "n_boxes", "n_uncertain", "n_infeasible")
+
+
+def certificate_record(cert):
+    """The certificate fields a history row keeps (a FeasibilityResult or a dict of its fields; None)."""
+    if cert is None:
+        return None
+    get = cert.get if isinstance(cert, dict) else (lambda k: getattr(cert, k))
+    return {k: get(k) for k in CERTIFICATE_FIELDS}
 
 
 def _protect(config, value):
@@ -247,7 +259,8 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                        gradient_norm=G, gradient_dot_direction=slope, actual_reduction=actual,
                        energy_resolution=resolution, certified_reduction=certified, energy_base=st.energy.total,
                        energy_trial=tr.energy.total if tr.energy is not None else None,
-                       unproven_cells=tr.unproven_cells, jd_root=root_info, resolution_exit=exit_info)
+                       unproven_cells=tr.unproven_cells, certificate=certificate_record(tr.certificate),
+                       jd_root=root_info, resolution_exit=exit_info)
             history.append(row)
             if observer is not None:
                 observer(row)

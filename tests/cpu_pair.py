@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from oracle import jd_root as ojr, poisson as opo, prescreen as ops, q1 as Q, spline as osp
+from oracle import certificate as ocert, jd_root as ojr, poisson as opo, q1 as Q, spline as osp
 from seq_vp_3d_amd.hotpath import Energies, TrialResult
 
 
@@ -128,11 +128,14 @@ class CpuPairState:
         inner = pts[1:-1, 1:-1, 1:-1]
         if np.any(pts < lo) or np.any(pts > hi) or np.any(inner <= lo) or np.any(inner >= hi):
             return TrialResult(False, "INVALID_DOMAIN", None)
-        low = ops.lower_bounds_c(u, self.grid, self.D)
-        unproven = int(np.sum(~(low > self.m.delta)))
         J = Q.geometry(u, self.grid, self.D)[2]
         if not J.min() > self.m.delta:
-            return TrialResult(False, "JACOBIAN_VIOLATION", None, unproven_cells=unproven)
+            return TrialResult(False, "JACOBIAN_VIOLATION", None)
+        cert = ocert.certify_map(u, self.grid.shape, self.grid.spacing, self.grid.axes, self.m.delta, D=self.D)
+        if cert["status"] != "certified_feasible":
+            reason = "FEASIBILITY_UNCERTAIN" if cert["status"] == "uncertain" else cert["reason"]
+            return TrialResult(False, reason, None, unproven_cells=cert["n_uncertain"] + cert["n_infeasible"],
+                               certificate=cert)
         value, grad = osp.sample(self.spline, pts)
         self._trial = (u, value, grad)
         e = self.evaluate_energy(u, value)
@@ -147,7 +150,7 @@ class CpuPairState:
         prox = float(np.sum(((y - x) * (y + x)).astype(np.longdouble)) / 2)
         total = img + prox + sdiff + bar[2]
         outer = img + bar[2]
-        return TrialResult(True, "", e, total, outer, min(bar[1], bar[3]) - self.m.delta, unproven)
+        return TrialResult(True, "", e, total, outer, min(bar[1], bar[3]) - self.m.delta, 0, None, cert)
 
     def promote(self, energy):
         self.theta = self.theta_t.clone()

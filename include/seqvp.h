@@ -316,6 +316,52 @@ int seqvp_certify_map_host(const double* u, int64_t nx, int64_t ny, int64_t nz, 
                            const double* az, const double* spacing, double delta, const double* D, int64_t tile,
                            int max_depth, int64_t max_boxes, int workers, SeqvpCertificate* out);
 
+/* ---- FP32 pyramid pre-optimiser (replaces preoptimize, jd_curl/three_d/preopt.py:518-614, and the analytic
+ *      gradient of preopt_grad.py) ----------------------------------------------------------------------------------
+ * Fields: u (3, nz-2, ny-2, nx-2) component-major, voxel units, the interior nodes of a zero-face field; images
+ * (nz, ny, nx). fp32 = 1: float32 buffers, 0: float64 (the error reference). */
+
+/* Device workspace of seqvp_preopt_energy_grad. */
+size_t seqvp_preopt_workspace_bytes(int64_t nx, int64_t ny, int64_t nz, int fp32);
+
+/* P(u) and dP/du (_pre_energy_tensors, preopt.py:157-202; preopt_grad.py:120-205): trilinear border-clamped data
+ * term, cell-centre (J - 1)^2 + |curl|^2 (weight mu/2), relu(j0 - J)^2 floor (weight kappa), seven-point curvature
+ * (weight alpha/2). terms (6 device doubles) = [total, data, geometry, floor, curvature, min J]; grad like u.
+ * Spacing (hx, hy, hz) of the level grid. */
+int seqvp_preopt_energy_grad(const void* u, const void* fixed, const void* moving, int64_t nx, int64_t ny, int64_t nz,
+                             double hx, double hy, double hz, double mu, double alpha, double kappa, double j0, int fp32,
+                             void* grad, double* terms, void* workspace, size_t workspace_bytes, void* stream);
+
+/* One update (the loop body of _run_level_default, preopt.py:219-233): grad smoothed by the separable kernel
+ * w_grad[0..2 r_grad] (r_grad < 0: no smoothing; replicate padding), Adam moments m, v, step scaled so that
+ * max |step| <= cap, u -= step, then u smoothed by w_field (r_field < 0: none). grad and tmp are overwritten.
+ * state (2 device int64): [t, cap hits], both advanced on the device (t = iterations done). pmax: 1024 scratch
+ * values of the field type. No host synchronisation: the launches can be captured in a graph. */
+int seqvp_preopt_step(void* u, void* m, void* v, void* grad, void* tmp, int64_t nx, int64_t ny, int64_t nz,
+                      const void* w_grad, int r_grad, const void* w_field, int r_field, double lr, double cap,
+                      double beta1, double beta2, double eps, int fp32, long long* state, void* pmax, void* stream);
+
+/* Pyramid resampling, float64 (downsample_array, nifti.py:128-149; prolong_displacement, multiresolution.py:55-84;
+ * boundary_taper, preopt.py:617-624). Arrays are C-order (n0, n1, n2, nc).
+ * seqvp_gauss_axis: correlation with w[0..2r] along axis 0/1/2, replicate padding (in != out).
+ * seqvp_lerp_axis: out[.., j, ..] = (1 - weight[j]) in[.., left[j], ..] + weight[j] in[.., right[j], ..], m outputs
+ * along the axis, separately rounded (bitwise the host prolongation). */
+int seqvp_gauss_axis(const double* in, double* out, int64_t n0, int64_t n1, int64_t n2, int64_t nc, int axis,
+                     const double* w, int r, void* stream);
+int seqvp_lerp_axis(const double* in, double* out, int64_t n0, int64_t n1, int64_t n2, int64_t nc, int axis, int64_t m,
+                    const int64_t* left, const int64_t* right, const double* weight, void* stream);
+/* out = ((rz[z] ry[y]) rx[x]) u for a (nz, ny, nx, 3) field. */
+int seqvp_taper(const double* u, int64_t nx, int64_t ny, int64_t nz, const double* rx, const double* ry,
+                const double* rz, double* out, void* stream);
+/* out = a u (n doubles). */
+int seqvp_scale(const double* u, double a, double* out, int64_t n, void* stream);
+/* (nz, ny, nx, 3) mm field <-> (3, nz-2, ny-2, nx-2) voxel-unit interior (float32 when fp32 = 1); the reverse
+ * writes zero faces. */
+int seqvp_field_to_interior(const double* u, int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                            int fp32, void* out, void* stream);
+int seqvp_interior_to_field(const void* v, int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
+                            int fp32, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,9 +59,20 @@ SIGNATURES = {
     "seqvp_certificate_finish": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_i64, c_int, c_i64, c_int, c_vp],
     "seqvp_certify_map_host": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_i64, c_int, c_i64,
                                c_int, c_vp],
+    "seqvp_preopt_workspace_bytes": [c_i64, c_i64, c_i64, c_int],
+    "seqvp_preopt_energy_grad": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+                                 c_dbl, c_int, c_vp, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_preopt_step": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_int, c_vp, c_int, c_dbl, c_dbl,
+                          c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp, c_vp],
+    "seqvp_gauss_axis": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_vp, c_int, c_vp],
+    "seqvp_lerp_axis": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "seqvp_taper": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "seqvp_scale": [c_vp, c_dbl, c_vp, c_i64, c_vp],
+    "seqvp_field_to_interior": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
+    "seqvp_interior_to_field": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
 }
 _RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz,
-             "seqvp_certificate_workspace_bytes": c_sz}
+             "seqvp_certificate_workspace_bytes": c_sz, "seqvp_preopt_workspace_bytes": c_sz}
 
 
 class Certificate(ctypes.Structure):

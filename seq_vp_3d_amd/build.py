@@ -2,7 +2,8 @@
 
 Every translation unit is compiled for gfx950 only. The prescreen and certificate units are compiled
 without floating-point contraction: the prescreen's proof bound assumes every +, -, * is rounded
-separately, and the interval certificate widens each separately rounded result by one ulp.
+separately, the interval certificate widens each separately rounded result by one ulp, and the pyramid
+resampling reproduces the separately rounded (1 - w) a + w b of the Q1 prolongation.
 Packed FP32 instructions are disabled for all units.
 """
 import os
@@ -16,10 +17,12 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libseqvp_hip.so"
 OBJ = HERE / "_build"
 
-SOURCES = ("q1_sweeps.hip", "prescreen.hip", "certificate.hip", "spline.hip", "dst.hip", "vec.hip", "abi.hip")
+SOURCES = ("q1_sweeps.hip", "prescreen.hip", "certificate.hip", "spline.hip", "dst.hip", "vec.hip", "preopt.hip",
+           "resample.hip", "abi.hip")
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-PER_FILE = {"prescreen.hip": ["-ffp-contract=off"], "certificate.hip": ["-ffp-contract=off"]}
+PER_FILE = {"prescreen.hip": ["-ffp-contract=off"], "certificate.hip": ["-ffp-contract=off"],
+            "resample.hip": ["-ffp-contract=off"]}
 
 
 def hipcc():

@@ -40,8 +40,14 @@ def parse():
     ap.add_argument("--inner", type=int, default=5)
     ap.add_argument("--shape", type=str, default="160,192,224")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("driver", "fixed"), default="driver")
+    ap.add_argument("--workload", choices=("driver", "fixed", "precision", "serve", "batch"), default="driver")
+    ap.add_argument("--requests", type=int, default=24, help="serve workload: number of requests")
+    ap.add_argument("--rate", type=float, default=2., help="serve workload: Poisson arrival rate (requests/s)")
+    ap.add_argument("--slots", type=int, default=4, help="serve/batch workloads: concurrent requests")
+    ap.add_argument("--start", choices=("zero", "half"), default="zero",
+                    help="initial map: u = 0 (the configuration's start) or half the known displacement")
     ap.add_argument("--fp32-screen", action="store_true", help="precision form: FP32 screening energies")
+    ap.add_argument("--outer", type=int, default=20, help="precision workload: outer steps of each FP64 solve")
     return ap.parse_args()
 
 
@@ -101,6 +107,128 @@ def cpu_baseline(shape_native, inner):
                        f"sub-volume ({g.n_cells} cells, {dt:.1f} s), scaled by the native cell count")
 
 
+def precision_form(args, pair, problem, dev):
+    """The precision form at the pair's shape: FP32 8/4/2/1 pyramid pre-optimisation (native flags), certified
+    transfer and `--outer` certified FP64 outer steps, beside `--outer` FP64 outer steps from u = 0. One JSON line
+    with the per-level FP32-vs-FP64 errors and the final energies of both routes."""
+    import numpy as np
+    import torch
+
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.preopt import PreoptConfig, preoptimized_solve
+    from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+    from seq_vp_3d_amd.spline import DeviceSpline
+
+    spline = DeviceSpline.from_image(pair.moving, pair.grid)
+    cfg = PreoptConfig.native()
+
+    def log(label):
+        return lambda row: print(f"{label} outer {row['outer_i']} {row['reason']} E {row['E_candidate']} "
+                                 f"{row['seconds'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # warm the kernels once (a short pre-optimisation on the same pair)
+    preoptimized_solve(problem, pair.fixed, pair.moving, dev, PreoptConfig(iterations=(2, 2, 2, 2)), max_outer=1,
+                       sampler=spline, report_errors=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, res, rec = preoptimized_solve(problem, pair.fixed, pair.moving, dev, cfg, max_outer=args.outer,
+                                      observer=log("preopt-route"), sampler=spline)
+    torch.cuda.synchronize()
+    route_s = time.perf_counter() - t0
+    if res is None:
+        raise SystemExit(f"no certified transfer: {rec['transfer']}")
+    st64 = PairState(problem, pair.fixed, pair.moving, dev, sampler=spline)
+    st64.set_state(theta=torch.zeros(st64.nfree, dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res64 = solve_sequential(st64, SequentialConfig(), PGConfig(), observer=log("fp64-only"), max_outer=args.outer)
+    torch.cuda.synchronize()
+    fp64_s = time.perf_counter() - t1
+    levels = [dict(level=lv["level"], shape_zyx=lv["shape_zyx"], iterations=lv["iterations"],
+                   seconds=round(lv["seconds"], 4), graph=lv["graph"], first_total=lv["first"]["total"],
+                   last_total=lv["last"]["total"], start_errors=lv["start_errors"], end_errors=lv["end_errors"])
+              for lv in rec["preopt"]["levels"]]
+    e_pre, e_64 = res.energy.total, res64.energy.total
+    line = dict(metric="precision form @%s" % "x".join(map(str, pair.grid.shape)), unit="s", dtype="f32+f64",
+                data="synthetic", config=dict(workload="FP32 8/4/2/1 pyramid pre-optimisation (native flags) + "
+                                                        "certified transfer + FP64 certified outer steps",
+                                              shape_zyx=list(pair.grid.shape), outer_steps=args.outer,
+                                              preopt=dict(iterations=list(cfg.iterations), lr=cfg.lr,
+                                                          sigma_field=cfg.sigma_field, mu=list(cfg.mu))),
+                preopt_seconds=round(rec["preopt_seconds"], 4), transfer_alpha=rec["transfer"]["alpha"],
+                transfer_seconds=round(rec["transfer"]["seconds"], 4), handoff_energy=rec["handoff_energy"],
+                levels=levels, final_energy_preopt_route=e_pre, final_energy_fp64_only=e_64,
+                final_energy_ratio=e_pre / e_64, route_seconds=round(route_s, 3), fp64_only_seconds=round(fp64_s, 3),
+                outer_status=dict(preopt_route=res.status, fp64_only=res64.status))
+    print(json.dumps(line), flush=True)
+
+
+SERVE_SIZES = ((96, 96, 96), (128, 128, 128), (160, 192, 144), (160, 192, 224))
+BATCH_SIZES = ((96, 96, 96),) * 8 + ((128, 128, 128),) * 8 + ((160, 192, 144),) * 4 + ((160, 192, 224),) * 4
+
+
+def serving_form(args, dev):
+    """The batching (all requests at t = 0) and serving (Poisson arrivals) forms against serial one-pair-at-a-time
+    execution of the same requests; each request runs `--outer` certified outer steps from u = 0 and its output is
+    re-certified on the CPU. One JSON line: certified pairs/hour of both, p50/p99 latency, and whether every
+    result is bitwise the same alone and in the mix."""
+    import numpy as np
+    import torch
+
+    from seq_vp_3d_amd.problem import make_grid_3d
+    from seq_vp_3d_amd.serving import Request, latency_summary, poisson_offsets, serial, serve
+    from seq_vp_3d_amd.synthetic import device_min_jd, device_sampler, make_pair, scaled_spec
+
+    if args.workload == "batch":
+        shapes, seeds = BATCH_SIZES, [1000 + i for i in range(len(BATCH_SIZES))]
+    else:
+        pick = np.random.default_rng(7).integers(0, len(SERVE_SIZES), size=args.requests)
+        shapes, seeds = [SERVE_SIZES[k] for k in pick], [2000 + i for i in range(args.requests)]
+    t0 = time.perf_counter()
+    reqs = []
+    for i, (shape, seed) in enumerate(zip(shapes, seeds)):
+        grid = make_grid_3d(nx=shape[2], ny=shape[1], nz=shape[0], hx=1., hy=1., hz=1.)
+        pair = make_pair(scaled_spec(shape, seed), dev, device_sampler(dev), device_min_jd(grid, dev))
+        reqs.append(Request(pair.fixed.cpu().pin_memory(), pair.moving.cpu().pin_memory(), max_outer=args.outer,
+                            request_id=i))
+        del pair
+    torch.cuda.synchronize()
+    print(f"generated {len(reqs)} pairs in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    # warm-up: every shape once (kernels, allocator pools, host caches)
+    warm = {}
+    for r in reqs:
+        warm.setdefault(r.shape, Request(r.fixed, r.moving, max_outer=1, request_id=-1))
+    serial(list(warm.values()), dev)
+    torch.cuda.synchronize()
+    alone, serial_s = serial(reqs, dev)
+    print(f"serial {serial_s:.2f}s", file=sys.stderr, flush=True)
+    offsets = None if args.workload == "batch" else poisson_offsets(len(reqs), args.rate, 7)
+    mixed, wall, server = serve(reqs, offsets, dev, slots=args.slots)
+    print(f"served {wall:.2f}s", file=sys.stderr, flush=True)
+    same = all(a.theta_digest == b.theta_digest and a.energy == b.energy for a, b in zip(alone, mixed))
+    recert = sum(r.recertified == "certified_feasible" for r in mixed)
+    lat_mix, lat_alone = latency_summary(mixed), latency_summary(alone)
+    n = len(reqs)
+    line = dict(metric="certified pairs/hour & p50/p99 latency served", unit="pairs/hour",
+                value=round(n / wall * 3600, 1), higher_is_better=True, n_gpus=1, dtype="f64", data="synthetic",
+                config=dict(workload=("batching: %d pairs of uneven sizes submitted at once" % n
+                                      if args.workload == "batch" else
+                                      "serving: %d requests, Poisson arrivals at %.2f/s" % (n, args.rate)),
+                            sizes={"x".join(map(str, k)): sum(1 for r in reqs if r.shape == k)
+                                   for k in sorted(set(r.shape for r in reqs))},
+                            outer_steps_per_request=args.outer, slots=args.slots,
+                            pair_seeds=f"{seeds[0]}..{seeds[-1]}"),
+                serial_pairs_per_hour=round(n / serial_s * 3600, 1), speedup_vs_serial=round(serial_s / wall, 3),
+                wall_seconds=round(wall, 3), serial_seconds=round(serial_s, 3),
+                latency_seconds=dict(mixed={k: round(v, 4) for k, v in lat_mix.items()},
+                                     alone={k: round(v, 4) for k, v in lat_alone.items()}),
+                max_concurrent=max(a[4] for a in server.admissions), recertified=f"{recert}/{n}",
+                same_alone_and_in_mix=same,
+                per_request=[dict(id=r.request_id, shape=list(r.shape), latency=round(r.latency, 4),
+                                  solve=round(r.seconds["solve"], 4), setup=round(r.seconds["setup"], 4),
+                                  recertify=round(r.seconds["recertify"], 4), status=r.status) for r in mixed])
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -113,24 +241,26 @@ def main():
     from seq_vp_3d_amd import _lib
     from seq_vp_3d_amd.hotpath import PairState
     from seq_vp_3d_amd.problem import Q1Problem, native_weights
-    from seq_vp_3d_amd.spline import DeviceSpline, build_image_sampler
-    from seq_vp_3d_amd.synthetic import CONFIG_NATIVE, PairSpec, make_pair
+    from seq_vp_3d_amd.problem import make_grid_3d
+    from seq_vp_3d_amd.synthetic import CONFIG_NATIVE, PairSpec, device_min_jd, device_sampler, make_pair
 
     dev = torch.device("cuda:0")
+    if args.workload in ("serve", "batch"):
+        return serving_form(args, dev)
     shape = tuple(int(v) for v in args.shape.split(","))
     spec = CONFIG_NATIVE if shape == CONFIG_NATIVE.shape else PairSpec(shape, CONFIG_NATIVE.blobs, CONFIG_NATIVE.sigma,
                                                                        CONFIG_NATIVE.modes, CONFIG_NATIVE.jmin)
 
-    def sampler(image, grid, points):
-        sp = DeviceSpline(build_image_sampler(image.cpu().numpy(), grid), grid, dev)
-        return sp.sample(points, with_gradient=False)[0]
-
     t0 = time.perf_counter()
-    pair = make_pair(spec, dev, sampler)
+    native_grid = make_grid_3d(nx=shape[2], ny=shape[1], nz=shape[0], hx=1., hy=1., hz=1.)
+    pair = make_pair(spec, dev, device_sampler(dev), device_min_jd(native_grid, dev))
     problem = Q1Problem(pair.grid, native_weights(shape), 0., .05)
+    if args.workload == "precision":
+        return precision_form(args, pair, problem, dev)
     st = PairState(problem, pair.fixed, pair.moving, dev)
     theta0 = torch.zeros(st.nfree, dtype=torch.float64, device=dev)
-    st.restrict((.5 * pair.displacement).contiguous(), theta0)
+    if args.start == "half":
+        st.restrict((.5 * pair.displacement).contiguous(), theta0)
     st.set_state(theta=theta0)
     setup_s = time.perf_counter() - t0
     print(f"setup {setup_s:.1f}s amplitude {pair.amplitude:.3f} E0 {st.energy.total:.6e}", file=sys.stderr, flush=True)
@@ -183,8 +313,8 @@ def main():
                               "refresh)") if args.workload == "driver" else
                              ("fixed outer-step sequence of the FP64 hot path (metric build + "
                               f"{args.inner} PG iterations, trial + prescreen + promotion each)"),
-                    shape_zyx=list(shape), pair="synthetic native recipe seed 1", trials=stats["trials"],
-                    accepted=stats["accepted"]),
+                    shape_zyx=list(shape), pair="synthetic native recipe seed 1", start=args.start,
+                    trials=stats["trials"], accepted=stats["accepted"]),
         roofline=dict(bound="mfma", achieved=round(achieved, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                       frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None, kernel=dominant,
                       mean_ms=round(mean_ms, 4), note="FP64 vector peak (equal to the FP64 matrix peak on MI355X); "

@@ -70,6 +70,7 @@ SIGNATURES = {
     "seqvp_scale": [c_vp, c_dbl, c_vp, c_i64, c_vp],
     "seqvp_field_to_interior": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
     "seqvp_interior_to_field": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
+    "seqvp_axis_transform": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_sz, c_vp],
 }
 _RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz,
              "seqvp_certificate_workspace_bytes": c_sz, "seqvp_preopt_workspace_bytes": c_sz}

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(CB) edge_hull_kernel(const double* __restrict_
   load_vertices(u, c, d, uv);
   const double h[3] = {hx, hy, hz};
   bool ok = true;
-  margin[c] = iv::edge_hull_margin(uv, h, delta, ok);
+  margin[c] = iv::edge_hull_margin<true>(uv, h, delta, ok);
   if (cell_flag) cell_flag[c] = ok ? 0 : 1;
   if (!ok && tile_flag) tile_flag[c / tile] = 1;   // every writer stores the same byte
 }

@@ -323,3 +323,25 @@ extern "C" int seqvp_poisson_denominator(const double* lx, const double* wx, con
                      wy, lz, wz, mx, my, mz, q1, omega, a, b, den);
   return launch_status();
 }
+
+// Separable linear map of a scalar node field (nz, ny, nx): out = (Mz (x) My (x) Mx) in, i.e. each axis line
+// multiplied by that axis's dense matrix (row-major, M[i][j] = coefficient i of unit input j). mx_t is Mx
+// TRANSPOSED (the x pass right-multiplies rows). Used for the not-a-knot cubic spline coefficients of an image
+// (the inverse collocation matrices of make_interp_spline, interpolation.py:29-40). FP64 MFMA passes:
+// x in -> out, y out -> ws, z ws -> out. ws holds nx*ny*nz doubles.
+extern "C" int seqvp_axis_transform(const double* in, double* out, const double* mx_t, const double* my,
+                                    const double* mz, int64_t nx, int64_t ny, int64_t nz, void* ws, size_t ws_bytes,
+                                    void* stream) {
+  if (!in || !out || !mx_t || !my || !mz || in == out || nx < 1 || ny < 1 || nz < 1 || nz > 65535) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < (size_t)(nx * ny * nz) * sizeof(double)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* tmp = (double*)ws;
+  const int64_t O = nz * ny;
+  hipLaunchKernelGGL((dst_rows_kernel<double, double, double, PRE_NONE, POST_NONE>), dim3(cdiv64(O, TM), cdiv64(nx, TN)),
+                     dim3(256), 0, s, in, nullptr, mx_t, O, nx, nullptr, 1.0, out);
+  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(nx, TN), cdiv64(ny, TM), nz), dim3(256), 0, s,
+                     out, my, ny, nx, nullptr, tmp);
+  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(ny * nx, TN), cdiv64(nz, TM), 1), dim3(256), 0,
+                     s, tmp, mz, nz, ny * nx, nullptr, out);
+  return launch_status();
+}

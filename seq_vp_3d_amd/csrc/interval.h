@@ -41,6 +41,15 @@ __host__ __device__ __forceinline__ double next_up(double x) {
 // nextafter(x, -inf).
 __host__ __device__ __forceinline__ double next_down(double x) { return -next_up(-x); }
 
+// Variants for finite operands: equal to next_up/next_down for every finite x; for a nonfinite x the result is
+// nonfinite too (NaN or inf), so an operation's validity check fails exactly where the full version's does.
+__host__ __device__ __forceinline__ double next_up_finite(double x) {
+  if (x == 0.0) return from_bits(1ull);
+  const uint64_t b = bits_of(x);
+  return from_bits(x > 0.0 ? b + 1 : b - 1);
+}
+__host__ __device__ __forceinline__ double next_down_finite(double x) { return -next_up_finite(-x); }
+
 // NumPy minimum/maximum: NaN propagates; otherwise the comparison result (ties return the second operand).
 __host__ __device__ __forceinline__ double np_min(double a, double b) {
   if (is_nan(a)) return a;
@@ -80,12 +89,34 @@ __host__ __device__ __forceinline__ I determinant(const I F[3][3], bool& ok) {
   return add(add(mul(F[0][0], a, ok), mul(F[0][1], b, ok), ok), mul(F[0][2], c, ok), ok);
 }
 
+// The same operations on finite operands (FAST: next_*_finite and plain min/max). Once any step of a cell is
+// nonfinite the cell is flagged by both versions; for an unflagged cell every intermediate is finite, where the
+// two versions agree bit for bit. The GPU screen uses FAST (fewer registers and branches).
+template <bool FAST>
+struct Ops {
+  static __host__ __device__ __forceinline__ double mn(double a, double b) { return FAST ? (a < b ? a : b) : np_min(a, b); }
+  static __host__ __device__ __forceinline__ double mx(double a, double b) { return FAST ? (a > b ? a : b) : np_max(a, b); }
+  static __host__ __device__ __forceinline__ I out(double lo, double hi, bool& ok) {
+    const I r = FAST ? I{next_down_finite(lo), next_up_finite(hi)} : I{next_down(lo), next_up(hi)};
+    ok = ok && is_finite(r.lo) && is_finite(r.hi) && r.lo <= r.hi;
+    return r;
+  }
+  static __host__ __device__ __forceinline__ I add(I a, I b, bool& ok) { return out(a.lo + b.lo, a.hi + b.hi, ok); }
+  static __host__ __device__ __forceinline__ I sub(I a, I b, bool& ok) { return out(a.lo - b.hi, a.hi - b.lo, ok); }
+  static __host__ __device__ __forceinline__ I mul(I a, I b, bool& ok) {
+    const double x = a.lo * b.lo, y = a.lo * b.hi, z = a.hi * b.lo, w = a.hi * b.hi;
+    return out(mn(mn(mn(x, y), z), w), mx(mx(mx(x, y), z), w), ok);
+  }
+};
+
 // Edge-hull margin of one cell (compiled_enclosure.py:193-227): F[r][j] = hull of the four raw edge
 // differences along j (widened once, then divided by h_j with one more outward step) plus the identity,
-// margin = lo(det F - delta). `ok` is false when any step was nonfinite or inverted (the cell's flag).
-// Vertex a = x + 2y + 4z; uv[a][r] is component r of the displacement at vertex a.
+// margin = lo(det F - delta) with the left-associated cofactor determinant. `ok` is false when any step was
+// nonfinite or inverted (the cell's flag). Vertex a = x + 2y + 4z; uv[a][r] is component r at vertex a.
+template <bool FAST = false>
 __host__ __device__ inline double edge_hull_margin(const double uv[8][3], const double h[3], double delta,
                                                    bool& ok) {
+  using O = Ops<FAST>;
   I F[3][3];
   for (int j = 0; j < 3; ++j) {
     for (int r = 0; r < 3; ++r) {
@@ -96,18 +127,21 @@ __host__ __device__ inline double edge_hull_margin(const double uv[8][3], const 
         const int b = a | (1 << j);
         const double d = uv[b][r] - uv[a][r];
         if (!is_finite(d)) good = false;
-        lower = np_min(lower, d);
-        upper = np_max(upper, d);
+        lower = O::mn(lower, d);
+        upper = O::mx(upper, d);
       }
-      I e = outward(lower, upper, good);
-      e = outward(e.lo / h[j], e.hi / h[j], good);
+      I e = O::out(lower, upper, good);
+      e = O::out(e.lo / h[j], e.hi / h[j], good);
       const double id = r == j ? 1.0 : 0.0;
-      F[r][j] = add(e, point(id), good);
+      F[r][j] = O::add(e, point(id), good);
       ok = ok && good;
     }
   }
-  const I J = determinant(F, ok);
-  const I m = outward(J.lo - delta, J.hi - delta, ok);
+  const I a = O::sub(O::mul(F[1][1], F[2][2], ok), O::mul(F[1][2], F[2][1], ok), ok);
+  const I b = O::sub(O::mul(F[1][2], F[2][0], ok), O::mul(F[1][0], F[2][2], ok), ok);
+  const I c = O::sub(O::mul(F[1][0], F[2][1], ok), O::mul(F[1][1], F[2][0], ok), ok);
+  const I J = O::add(O::add(O::mul(F[0][0], a, ok), O::mul(F[0][1], b, ok), ok), O::mul(F[0][2], c, ok), ok);
+  const I m = O::out(J.lo - delta, J.hi - delta, ok);
   return m.lo;
 }
 

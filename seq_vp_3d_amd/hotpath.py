@@ -27,7 +27,7 @@ from . import _lib
 from .poisson import DevicePoisson, SpectralMetric
 from .certificate import CertificateConfig, DeviceCertificate
 from .q1 import FusedJDRoot, FusedQ1
-from .spline import DeviceSpline, build_image_sampler
+from .spline import DeviceSpline
 
 JD_AFFINE_FLOOR = 1e-12
 JD_AFFINE_ACTIVATION = .25
@@ -83,8 +83,12 @@ class PairState:
         t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous()
         self.fixed = t(fixed)
         self.data_weight = None if data_weight is None else t(data_weight)
-        mov = moving.detach().cpu().numpy() if isinstance(moving, torch.Tensor) else np.asarray(moving)
-        self.spline = DeviceSpline(sampler if sampler is not None else build_image_sampler(mov, g), g, self.device)
+        if isinstance(sampler, DeviceSpline):           # shared device spline of the same moving image
+            self.spline = sampler
+        elif sampler is not None:                       # host coefficients (build_image_sampler)
+            self.spline = DeviceSpline(sampler, g, self.device)
+        else:                                           # coefficients computed on the device
+            self.spline = DeviceSpline.from_image(t(moving), g, self.device)
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)
         self.ref = z(*self.shape, 3)                 # u_ref (proximal centre)
         self.u = z(*self.shape, 3)                   # accepted state

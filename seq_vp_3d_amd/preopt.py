@@ -370,3 +370,32 @@ def certify_transfer(st, seed, alphas=TRANSFER_ALPHAS):
         if reason not in ("JACOBIAN_VIOLATION", "subdivision_budget", "INVALID_DOMAIN"):
             break
     return None, attempts
+
+
+def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_outer=None, observer=None,
+                       sampler=None, report_errors=True):
+    """The precision form: FP32 pyramid pre-optimisation, boundary taper, certified transfer, then the certified
+    FP64 solve (solve_sequential) from the transferred state (the pre-optimizer route of
+    scripts/run_stage_3d_e19_mri.py:930-960 and multiresolution.prepare_transfer_problem).
+    Returns (PairState, SequentialResult or None when no alpha certified, record)."""
+    from .hotpath import PairState
+    from .solver import PGConfig, SequentialConfig, solve_sequential
+    device = torch.device(device)
+    config = PreoptConfig.native() if config is None else config
+    t0 = perf_counter()
+    u, rec = PreOptimizer(device, config)(fixed, moving, problem.grid, report_errors)
+    seed = taper(u, problem.grid, config.taper)
+    torch.cuda.synchronize(device)
+    t1 = perf_counter()
+    st = PairState(problem, fixed, moving, device, sampler=sampler)
+    alpha, attempts = certify_transfer(st, seed)
+    torch.cuda.synchronize(device)
+    t2 = perf_counter()
+    record = dict(preopt=rec, preopt_seconds=t1 - t0, transfer=dict(alpha=alpha, attempts=attempts,
+                                                                    seconds=t2 - t1))
+    if alpha is None:
+        return st, None, record
+    record["handoff_energy"] = st.energy.total
+    res = solve_sequential(st, SequentialConfig(), PGConfig(), observer=observer, max_outer=max_outer)
+    record["solve_seconds"] = perf_counter() - t2
+    return st, res, record

@@ -2,10 +2,14 @@
 
 build_image_sampler follows jd_curl/three_d/interpolation.py:29-40: the coefficients come from
 SciPy's make_interp_spline (k=3, not-a-knot) along z, y, x in turn -- the reference's own
-third-party routine, run once per moving image on the host. Sampling (value and physical
-gradient at X + u or at arbitrary points) runs in libseqvp_hip.so, as TorchSpline.sample does
-(backends/spline.py:81-127).
+third-party routine, on the host. DeviceSpline.from_image computes the same coefficients on the
+device: the interpolant is linear in the image, so each axis pass is one dense matrix (the
+inverse collocation matrix, make_interp_spline of the identity, n <= a few hundred) applied to
+every line along that axis with the FP64 MFMA contraction of seqvp_axis_transform. Sampling
+(value and physical gradient at X + u or at arbitrary points) runs in libseqvp_hip.so, as
+TorchSpline.sample does (backends/spline.py:81-127).
 """
+from functools import lru_cache
 from dataclasses import dataclass
 
 import numpy as np
@@ -36,14 +40,53 @@ def build_image_sampler(image, grid):
     return SplineCoefficients(np.ascontiguousarray(coefficients), tuple(knots))
 
 
+@lru_cache(maxsize=16)
+def axis_operators(axes_key):
+    """(matrices, knots) per axis z, y, x: the not-a-knot cubic coefficient map of one line (make_interp_spline of
+    the identity: column j = the coefficients of unit data j) and the knot vector. axes_key = the axis
+    coordinates as tuples (z, y, x)."""
+    mats, knots = [], []
+    for coords in axes_key:
+        coords = np.asarray(coords, dtype=np.float64)
+        spline = make_interp_spline(coords, np.eye(len(coords)), k=3, axis=0, bc_type="not-a-knot")
+        mats.append(np.ascontiguousarray(spline.c, dtype=np.float64))
+        knots.append(np.array(spline.t, dtype=np.float64))
+    return tuple(mats), tuple(knots)
+
+
 class DeviceSpline:
     """Device-resident coefficients and knots of one moving image on one grid."""
 
     def __init__(self, sampler, grid, device="cuda"):
         self.grid = grid
         self.device = torch.device(device)
+        if sampler is None:          # filled by from_image
+            return
         self.c = torch.tensor(sampler.coefficients, dtype=torch.float64, device=self.device)
         self.t = tuple(torch.tensor(t, dtype=torch.float64, device=self.device) for t in sampler.knots)
+        self._finish()
+
+    @classmethod
+    def from_image(cls, image, grid, device=None):
+        """The spline of a device image (nz, ny, nx) float64 with the coefficients computed on the device."""
+        device = image.device if device is None else torch.device(device)
+        if tuple(image.shape) != tuple(grid.shape) or image.dtype != torch.float64:
+            raise InputValidationError("image must be a float64 device tensor of the grid shape")
+        self = cls(None, grid, device)
+        mats, knots = axis_operators(tuple(tuple(a) for a in grid.axes[::-1]))
+        mz, my, mx = (torch.as_tensor(m, device=device) for m in mats)
+        mx_t = mx.t().contiguous()
+        nz, ny, nx = grid.shape
+        self.c = torch.empty(grid.shape, dtype=torch.float64, device=device)
+        ws = torch.empty(grid.shape, dtype=torch.float64, device=device)
+        _lib.call("seqvp_axis_transform", _lib.ptr(image.contiguous()), _lib.ptr(self.c), _lib.ptr(mx_t),
+                  _lib.ptr(my), _lib.ptr(mz), nx, ny, nz, _lib.ptr(ws), ws.numel() * 8, _lib.stream_handle(device))
+        self.t = tuple(torch.as_tensor(t, device=device) for t in knots)
+        self._finish()
+        return self
+
+    def _finish(self):
+        grid = self.grid
         self.status = torch.zeros(1, dtype=torch.int64, device=self.device)
         x, y, z = grid.axes
         self.box = (float(x[0]), float(x[-1]), float(y[0]), float(y[-1]), float(z[0]), float(z[-1]))

@@ -105,22 +105,40 @@ class Pair:
     spec: PairSpec
 
 
-def make_pair(spec, device="cpu", sampler=None):
+def device_min_jd(grid, device):
+    """u -> min over cells and closed27 points of det(I + grad u), by the device barrier kernel (the min J output
+    of FusedQ1.barrier_energy); the fast replacement of closed27_min_jd on a GPU."""
+    from .problem import Q1Problem, Weights3D
+    from .q1 import FusedQ1
+    q = FusedQ1(Q1Problem(grid, Weights3D(1., 1., 1., 1.), 0., .05), device)
+    return lambda u: float(q.barrier_energy(u.contiguous())[1].item())
+
+
+def device_sampler(device):
+    """sampler(image, grid, points) with the spline coefficients built on the device."""
+    from .spline import DeviceSpline
+    return lambda image, grid, points: DeviceSpline.from_image(image.contiguous(), grid, device).sample(
+        points, with_gradient=False)[0]
+
+
+def make_pair(spec, device="cpu", sampler=None, min_jd=None):
     """Builds the pair. `sampler(image, grid, points)` returns the spline value of `image` at
-    `points` (the caller passes the device sampler on a GPU, or a host one on the CPU)."""
+    `points` (the caller passes the device sampler on a GPU, or a host one on the CPU); `min_jd(u)`
+    (default closed27_min_jd) is the closed27 min J used by the amplitude bisection."""
     nz, ny, nx = spec.shape
     grid = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=1., hy=1., hz=1.)
+    jd = (lambda u: closed27_min_jd(u, grid)) if min_jd is None else min_jd
     rng = np.random.default_rng(spec.seed)
     fixed = _blobs(spec, grid, rng, device)
     w = _modes(spec, grid, rng, device)
     lo, hi = 0., 1.
-    while closed27_min_jd(hi * w, grid) >= spec.jmin:
+    while jd(hi * w) >= spec.jmin:
         hi *= 2.
         if hi > 1e6:
             break
     for _ in range(40):
         mid = .5 * (lo + hi)
-        if closed27_min_jd(mid * w, grid) >= spec.jmin:
+        if jd(mid * w) >= spec.jmin:
             lo = mid
         else:
             hi = mid

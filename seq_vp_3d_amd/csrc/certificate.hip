@@ -884,7 +884,7 @@ extern "C" int seqvp_certify_map_host(const double* u, int64_t nx, int64_t ny, i
         uv[a][2] = p[2];
       }
       bool ok = true;
-      margin[(size_t)c] = iv::edge_hull_margin(uv, spacing, delta, ok);
+      margin[(size_t)c] = iv::edge_hull_margin<true>(uv, spacing, delta, ok);   // unflagged margins: bitwise
       flag[(size_t)c] = ok ? 0 : 1;
     }
   });

@@ -30,6 +30,19 @@ FLOPS_PER_UNIT = {
     "seqvp_spline_warp": 1078.,
 }
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch at 160x192x224 from the FETCH_SIZE and WRITE_SIZE counter runs (rocprofv3 --pmc, one counter
+# per run, profiles/hbm_traffic_native_pmc.md), each counter scaled by its reading for a copy of known size
+# (--copy-probe: fetch x2.0, write x1.0)
+TRAFFIC_BYTES_NATIVE = {
+    "seqvp_q1_soft_energy": 986e6,
+    "seqvp_q1_barrier_energy": 365e6,
+    "seqvp_q1_gradient": 3667e6,
+    "seqvp_q1_joint": 6359e6,
+    "seqvp_jd_root": 369e6,
+    "seqvp_certificate_screen": 282e6,
+    "seqvp_spline_warp": 532e6,
+    "seqvp_dst_poisson": 5928e6,
+}
 
 
 def parse():
@@ -48,6 +61,9 @@ def parse():
                     help="initial map: u = 0 (the configuration's start) or half the known displacement")
     ap.add_argument("--fp32-screen", action="store_true", help="precision form: FP32 screening energies")
     ap.add_argument("--outer", type=int, default=20, help="precision workload: outer steps of each FP64 solve")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="before the workload, one seqvp_scale launch over 1 GiB (reads 1 GiB, writes 1 GiB): the "
+                         "plain copy of known size that calibrates the FETCH_SIZE/WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -245,6 +261,14 @@ def main():
     from seq_vp_3d_amd.synthetic import CONFIG_NATIVE, PairSpec, device_min_jd, device_sampler, make_pair
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        n = 1 << 27                                   # 2^27 doubles = 1 GiB
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        _lib.call("seqvp_scale", _lib.ptr(a), 2., _lib.ptr(b), n, _lib.stream_handle(dev))
+        torch.cuda.synchronize()
+        del a, b
     if args.workload in ("serve", "batch"):
         return serving_form(args, dev)
     shape = tuple(int(v) for v in args.shape.split(","))
@@ -316,7 +340,9 @@ def main():
                     shape_zyx=list(shape), pair="synthetic native recipe seed 1", start=args.start,
                     trials=stats["trials"], accepted=stats["accepted"]),
         roofline=dict(bound="mfma", achieved=round(achieved, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                      frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None, kernel=dominant,
+                      frac=round(achieved / FP64_PEAK_TFLOPS, 4),
+                      traffic=TRAFFIC_BYTES_NATIVE.get(dominant) if shape == (160, 192, 224) else None,
+                      kernel=dominant,
                       mean_ms=round(mean_ms, 4), note="FP64 vector peak (equal to the FP64 matrix peak on MI355X); "
                                                         "achieved = algorithmic flops per launch / mean launch time"),
         kernels={k: dict(calls=c, mean_ms=round(ms, 4)) for k, (c, ms) in sorted(kern.items())},

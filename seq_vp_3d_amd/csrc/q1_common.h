@@ -88,6 +88,13 @@ __device__ __forceinline__ void make_edges(const T v[8][3], const T ih[3], Edges
         E.e[j][r][k] = (v[edge_a1(j, k)][r] - v[edge_a0(j, k)][r]) * ih[j];
 }
 
+// The bilinear blend of four parallel edges with explicit fused multiply-adds, so the derivative of a field is
+// the same bits whichever storage its edges come from (J(u) - J(ref) is exactly 0 when u == ref).
+template <typename T>
+__device__ __forceinline__ T blend4(T e0, T e1, T e2, T e3, T w00, T w10, T w01, T w11) {
+  return fma(e0, w00, e1 * w10) + fma(e2, w01, e3 * w11);
+}
+
 // V[r][j] = sum_k e[j][r][k] f(k0, s_o1) f(k1, s_o2) at reference point s = (sx, sy, sz).
 template <typename T>
 __device__ __forceinline__ void point_grad(const EdgesT<T>& E, const T s[3], T V[3][3]) {
@@ -98,18 +105,52 @@ __device__ __forceinline__ void point_grad(const EdgesT<T>& E, const T s[3], T V
     const T w01 = ((T)1 - so1) * so2, w11 = so1 * so2;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
-      V[r][j] = (E.e[j][r][0] * w00 + E.e[j][r][1] * w10) + (E.e[j][r][2] * w01 + E.e[j][r][3] * w11);
+      V[r][j] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], w00, w10, w01, w11);
+  }
+}
+
+typedef __attribute__((address_space(3))) double lds_double;
+
+// Edges of one field kept in LDS as a per-thread column (element (j, r, k) at col[((j*3 + r)*4 + k)*stride]):
+// an extension of the register file for kernels that hold several fields' edges at once. Each thread reads
+// only what it wrote, so no barrier is needed.
+__device__ __forceinline__ void stage_edges(const double v[8][3], const double ih[3], lds_double* col, int stride) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        col[((j * 3 + r) * 4 + k) * stride] = (v[edge_a1(j, k)][r] - v[edge_a0(j, k)][r]) * ih[j];
+}
+
+// volatile: the column must be re-read at every point (a hoisted copy would bring the registers back); the LDS
+// address space keeps the reads ds_read.
+__device__ __forceinline__ void point_grad_col(const volatile lds_double* col, int stride, const double s[3],
+                                               double V[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double so1 = s[other1(j)], so2 = s[other2(j)];
+    const double w00 = (1.0 - so1) * (1.0 - so2), w10 = so1 * (1.0 - so2);
+    const double w01 = (1.0 - so1) * so2, w11 = so1 * so2;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const volatile lds_double* e = col + (j * 3 + r) * 4 * stride;
+      V[r][j] = blend4<double>(e[0], e[stride], e[2 * stride], e[3 * stride], w00, w10, w01, w11);
+    }
   }
 }
 
 // F = I + V; cofactor rows A_r = F_{r+1} x F_{r+2}; J = F_0 . A_0 (geometry.py:11-23, 60-89).
+// Explicit fused multiply-adds: the same bits at every call site (the compiler's own contraction may fuse either
+// product of a difference, differently where the operands come from different storage).
 template <typename T>
 __device__ __forceinline__ T jacobian(const T V[3][3]) {
   const T f00 = V[0][0] + (T)1, f11 = V[1][1] + (T)1, f22 = V[2][2] + (T)1;
-  const T a00 = f11 * f22 - V[1][2] * V[2][1];
-  const T a01 = V[1][2] * V[2][0] - V[1][0] * f22;
-  const T a02 = V[1][0] * V[2][1] - f11 * V[2][0];
-  return (f00 * a00 + V[0][1] * a01) + V[0][2] * a02;
+  const T a00 = fma(f11, f22, -(V[1][2] * V[2][1]));
+  const T a01 = fma(V[1][2], V[2][0], -(V[1][0] * f22));
+  const T a02 = fma(V[1][0], V[2][1], -(f11 * V[2][0]));
+  return fma(V[0][2], a02, fma(f00, a00, V[0][1] * a01));
 }
 
 __device__ __forceinline__ double cofactor_jacobian(const double V[3][3], double A[3][3]) {

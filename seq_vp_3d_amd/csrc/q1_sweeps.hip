@@ -100,7 +100,7 @@ __device__ __forceinline__ void closed_point(int p, double s[3]) {
 // Soft JD/curl change energy (residuals.py:166-172) and its stable difference against a base
 // (energy_comparison.py:196-200, blocks as in residuals.py:96-103).
 template <bool WITH_BASE>
-__global__ void __launch_bounds__(BLOCK) soft_energy_kernel(const double* __restrict__ u, const double* __restrict__ ref,
+__global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __restrict__ u, const double* __restrict__ ref,
                                                             const double* __restrict__ base,
                                                             const double* __restrict__ k, Dims d,
                                                             double* __restrict__ partials) {
@@ -108,18 +108,21 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_kernel(const double* __rest
   const int64_t c = blk * BLOCK + threadIdx.x;
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   double none[1] = {0.0};
+  // the reference field's edges live in LDS (register pressure: two fields' edges stay in VGPRs)
+  __shared__ double er_lds[36 * BLOCK];
+  lds_double* er = (lds_double*)(er_lds + threadIdx.x);
   if (c < d.ncell) {
     double ih[3];
     load_ih(k, ih);
     const double mu = k[K_MU], nu = k[K_NU];
     const int64_t b = cell_base_node(c, d);
-    Edges Eu, Er, Eb;
+    Edges Eu, Eb;
     {
       double v[8][3];
       load_cell(u, b, d, v);
       make_edges(v, ih, Eu);
       load_cell(ref, b, d, v);
-      make_edges(v, ih, Er);
+      stage_edges(v, ih, er, BLOCK);
       if (WITH_BASE) {
         load_cell(base, b, d, v);
         make_edges(v, ih, Eb);
@@ -135,7 +138,7 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_kernel(const double* __rest
       point_grad(Eu, sp, V);
       const double J = jacobian(V);
       curl_of(V, C);
-      point_grad(Er, sp, W);
+      point_grad_col(er, BLOCK, sp, W);
       const double Jr = jacobian(W);
       curl_of(W, Cr);
       const double dj = J - Jr;
@@ -143,18 +146,18 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_kernel(const double* __rest
       cj += qw * dj * dj;
       cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
       if (WITH_BASE) {
+        // 1/2 w (y - x)(y + x) with y, x the trial and base residuals against the reference: the same stable
+        // difference as with sqrt(w) folded into the residuals; the point's four terms are summed first and
+        // added to the double-double accumulator once.
         double Y[3][3], Ca[3];
         point_grad(Eb, sp, Y);
         const double Ja = jacobian(Y);
         curl_of(Y, Ca);
-        const double sm = sqrt(mu * qw), sn = sqrt(nu * qw);
-        const double xj = sm * (Ja - Jr), yj = sm * dj;
-        dd_add(s[2], 0.5 * ((yj - xj) * (yj + xj)));
+        const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
+        double t = wm * ((J - Ja) * (dj + (Ja - Jr)));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double xc = sn * (Ca[i] - Cr[i]), yc = sn * (C[i] - Cr[i]);
-          dd_add(s[2], 0.5 * ((yc - xc) * (yc + xc)));
-        }
+        for (int i = 0; i < 3; ++i) t += wn * ((C[i] - Ca[i]) * ((C[i] - Cr[i]) + (Ca[i] - Cr[i])));
+        dd_add(s[2], t);
       }
     }
     dd_add(s[0], 0.5 * mu * cj);
@@ -387,7 +390,8 @@ __device__ __forceinline__ void adj_store(const EdgeAdj& G, const double ih[3], 
     }
 }
 
-__device__ __forceinline__ void soft_vjp(const Edges& Eu, const Edges& Er, const double* __restrict__ k, EdgeAdj& G) {
+__device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, int stride, const double* __restrict__ k,
+                                         EdgeAdj& G) {
   const double mu = k[K_MU], nu = k[K_NU];
 #pragma unroll 1
   for (int p = 0; p < 27; ++p) {
@@ -398,7 +402,7 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const Edges& Er, const
     point_grad(Eu, sp, V);
     const double J = cofactor_jacobian(V, A);
     curl_of(V, C);
-    point_grad(Er, sp, W);
+    point_grad_col(er, stride, sp, W);
     const double Jr = jacobian(W);
     curl_of(W, Cr);
     const double jd = mu * qw * (J - Jr);
@@ -444,7 +448,7 @@ __device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __res
 // MODE 0: soft (+ barrier) into one buffer; MODE 1: barrier only; MODE 2: soft into loc, barrier into
 // loc2 (the joint GF/GE form, linearization.py:217-238).
 template <int MODE, bool SOFT, bool BARRIER>
-__global__ void __launch_bounds__(BLOCK) grad_locals_kernel(const double* __restrict__ u, const double* __restrict__ ref,
+__global__ void __launch_bounds__(BLOCK, 2) grad_locals_kernel(const double* __restrict__ u, const double* __restrict__ ref,
                                                             const double* __restrict__ k, Dims d,
                                                             double* __restrict__ loc, double* __restrict__ loc2) {
   const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
@@ -461,11 +465,13 @@ __global__ void __launch_bounds__(BLOCK) grad_locals_kernel(const double* __rest
   EdgeAdj G;
   adj_zero(G);
   if (SOFT) {
-    Edges Er;
+    // the reference field's edges live in LDS (register pressure)
+    __shared__ double er_lds[36 * BLOCK];
+    lds_double* er = (lds_double*)(er_lds + threadIdx.x);
     double v[8][3];
     load_cell(ref, b, d, v);
-    make_edges(v, ih, Er);
-    soft_vjp(Eu, Er, k, G);
+    stage_edges(v, ih, er, BLOCK);
+    soft_vjp(Eu, er, BLOCK, k, G);
   }
   if (MODE == 2) {
     adj_store(G, ih, loc, c, d.ncell);

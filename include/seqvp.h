@@ -220,7 +220,7 @@ int seqvp_mfma_probe(const double* a, const double* b, int fp32, double* d, void
 /* Separable linear map of a scalar node field (nz, ny, nx): out = (Mz (x) My (x) Mx) in, each axis line multiplied by
  * that axis's dense row-major matrix; mx_t is Mx transposed. With the inverse collocation matrices of the not-a-knot
  * cubic interpolant it computes the image spline coefficients of build_image_sampler (interpolation.py:29-40,
- * make_interp_spline along z, y, x) on the device (FP64 MFMA). ws: nx*ny*nz doubles; in != out; nz <= 65535. */
+ * make_interp_spline along z, y, x) on the device (FP64 MFMA). ws: nx*ny*nz doubles; in != out. */
 int seqvp_axis_transform(const double* in, double* out, const double* mx_t, const double* my, const double* mz,
                          int64_t nx, int64_t ny, int64_t nz, void* ws, size_t ws_bytes, void* stream);
 

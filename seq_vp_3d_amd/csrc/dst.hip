@@ -70,7 +70,11 @@ __global__ void __launch_bounds__(256) dst_rows_kernel(const TIN* __restrict__ X
   typedef typename Acc<T>::type AT;
   __shared__ T As[TK][TM + 1];   // As[k][m]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * TM, n0 = (int64_t)blockIdx.y * TN;
+  // 1-D grid over (row tile, column tile), column tile fastest; each XCD takes a contiguous range of tiles so the
+  // column tiles that re-read the same X rows run back to back on one L2.
+  const int64_t ncb = (n + TN - 1) / TN;
+  const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t m0 = (tile / ncb) * TM, n0 = (tile % ncb) * TN;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   AT acc[2][2];
 #pragma unroll
@@ -144,8 +148,13 @@ __global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, 
   typedef typename Acc<T>::type AT;
   __shared__ T Bs[TK][TN + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t o = blockIdx.z;
-  const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+  // 1-D grid over (batch, inner tile, output-row tile), output-row tile fastest, XCD-contiguous (the output-row
+  // tiles of one inner tile re-read the same X columns).
+  const int64_t nmb = (n + TM - 1) / TM, nib = (I + TN - 1) / TN;
+  const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t o = tile / (nmb * nib);
+  const int64_t rest = tile % (nmb * nib);
+  const int64_t m0 = (rest % nmb) * TM, n0 = (rest / nmb) * TN;
   const T* Xo = X + o * n * I;
   T* Yo = Y + o * n * I;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
@@ -264,7 +273,7 @@ static int dst_poisson(const double* rhs, const double* den, const double* scale
   if (!rhs || !den || !out || mx < 1 || my < 1 || mz < 1 || ncomp < 1 || ncomp > 65535) return SEQVP_EINVAL;
   if (scale == out) return SEQVP_EINVAL;
   if (!ws || ws_bytes < seqvp_dst_workspace_bytes(mx, my, mz, ncomp)) return SEQVP_EWORKSPACE;
-  if (ncomp * mz > 65535) return SEQVP_EINVAL;
+  if (cdiv64(mx, TN) * cdiv64(my, TM) * ncomp * mz > 0x7fffffff) return SEQVP_EINVAL;
   hipStream_t s = (hipStream_t)stream;
   const int64_t total = ncomp * mx * my * mz;
   T* tmp = (T*)ws;
@@ -277,7 +286,7 @@ static int dst_poisson(const double* rhs, const double* den, const double* scale
   hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(mz * mz, 256)), dim3(256), 0, s, Sz, mz);
   const int64_t O = ncomp * mz * my;
   const double sign = negate ? -1.0 : 1.0;
-  const dim3 grow(cdiv64(O, TM), cdiv64(mx, TN));
+  const dim3 grow(cdiv64(O, TM) * cdiv64(mx, TN));
   // forward x (pre-scale): rhs -> tmp
   if (scale)
     hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_SCALE, POST_NONE>), grow, dim3(256), 0, s, rhs, scale, Sx, O,
@@ -286,10 +295,10 @@ static int dst_poisson(const double* rhs, const double* den, const double* scale
     hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_NONE, POST_NONE>), grow, dim3(256), 0, s, rhs, nullptr, Sx, O,
                        mx, nullptr, 1.0, tmp);
   // forward y: tmp -> alt (batch ncomp*mz, inner mx)
-  const dim3 gy(cdiv64(mx, TN), cdiv64(my, TM), ncomp * mz);
+  const dim3 gy(cdiv64(mx, TN) * cdiv64(my, TM) * ncomp * mz);
   hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gy, dim3(256), 0, s, tmp, Sy, my, mx, nullptr, alt);
   // forward z + division by the spectrum: alt -> tmp (batch ncomp, inner my*mx)
-  const dim3 gz(cdiv64(my * mx, TN), cdiv64(mz, TM), ncomp);
+  const dim3 gz(cdiv64(my * mx, TN) * cdiv64(mz, TM) * ncomp);
   hipLaunchKernelGGL((dst_cols_kernel<T, POST_DIV>), gz, dim3(256), 0, s, alt, Sz, mz, my * mx, den, tmp);
   // inverse z: tmp -> alt
   hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gz, dim3(256), 0, s, tmp, Sz, mz, my * mx, nullptr, alt);
@@ -332,16 +341,16 @@ extern "C" int seqvp_poisson_denominator(const double* lx, const double* wx, con
 extern "C" int seqvp_axis_transform(const double* in, double* out, const double* mx_t, const double* my,
                                     const double* mz, int64_t nx, int64_t ny, int64_t nz, void* ws, size_t ws_bytes,
                                     void* stream) {
-  if (!in || !out || !mx_t || !my || !mz || in == out || nx < 1 || ny < 1 || nz < 1 || nz > 65535) return SEQVP_EINVAL;
+  if (!in || !out || !mx_t || !my || !mz || in == out || nx < 1 || ny < 1 || nz < 1) return SEQVP_EINVAL;
   if (!ws || ws_bytes < (size_t)(nx * ny * nz) * sizeof(double)) return SEQVP_EWORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   double* tmp = (double*)ws;
   const int64_t O = nz * ny;
-  hipLaunchKernelGGL((dst_rows_kernel<double, double, double, PRE_NONE, POST_NONE>), dim3(cdiv64(O, TM), cdiv64(nx, TN)),
+  hipLaunchKernelGGL((dst_rows_kernel<double, double, double, PRE_NONE, POST_NONE>), dim3(cdiv64(O, TM) * cdiv64(nx, TN)),
                      dim3(256), 0, s, in, nullptr, mx_t, O, nx, nullptr, 1.0, out);
-  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(nx, TN), cdiv64(ny, TM), nz), dim3(256), 0, s,
+  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(nx, TN) * cdiv64(ny, TM) * nz), dim3(256), 0, s,
                      out, my, ny, nx, nullptr, tmp);
-  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(ny * nx, TN), cdiv64(nz, TM), 1), dim3(256), 0,
+  hipLaunchKernelGGL((dst_cols_kernel<double, POST_NONE>), dim3(cdiv64(ny * nx, TN) * cdiv64(nz, TM)), dim3(256), 0,
                      s, tmp, mz, nz, ny * nx, nullptr, out);
   return launch_status();
 }

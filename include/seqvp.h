@@ -106,6 +106,26 @@ int seqvp_q1_ggn_diagonal(const double* node_base, const double* u, const double
 int seqvp_node_min_jd(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz, double* node,
                       void* workspace, size_t workspace_bytes, void* stream);
 
+/* The same measure with the T69 face/interior extension (replaces jd_affine.node_min_jd with
+ * extension_activation, jd_affine.py:154-225, and the backend's node_min_jd(lin, extension_activation=...,
+ * stats=...), hybrid.py:97-101): on every cell whose Bernstein hull reaches below min(m_c, activation),
+ * m_c <- min(m_c, mu_c) with mu_c the face/interior estimate of seqvp_face_interior_minima.
+ * activation in (0, 1]. stats (device, 4 doubles, may be NULL): number of candidate cells, cells lowered by
+ * more than 0.1 %, cells lowered at all, least mu (+inf without candidates). Workspace:
+ * seqvp_node_min_jd_extended_workspace_bytes. */
+size_t seqvp_node_min_jd_extended_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+int seqvp_node_min_jd_extended(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz,
+                               double activation, double* node, double* stats, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* T69 face/interior estimate per cell (replaces jd_affine.face_interior_minima, jd_affine.py:130-151):
+ * J27 holds n rows of 27 closed-point Jacobians (index 9 iz + 3 iy + ix); mu[i] = least of the row's
+ * triquadratic on the 9^3 grid and of bounded Newton from its 4 lowest grid points. Device pointers. */
+int seqvp_face_interior_minima(const double* J27, int64_t n, double* mu, void* stream);
+/* HOST: the same estimate of host rows on the calling thread (the reference's CPU estimator,
+ * jd_affine.py:163-172). */
+int seqvp_face_interior_minima_host(const double* J27, int64_t n, double* mu);
+
 /* T66 global closed27 first JD root (replaces FusedJDRoot.__call__, triton_jd_root.py:38-52;
  * global_jd_step.py:22-129). u: base field, v: direction field (both node fields), limit > 0.
  * out[0] = min root in s-units (inf if none), out[1] = number of points with c0 <= 0 (as double),

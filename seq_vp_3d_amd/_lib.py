@@ -28,6 +28,10 @@ SIGNATURES = {
     "seqvp_q1_joint": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_ggn_diagonal": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_node_min_jd": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_node_min_jd_extended_workspace_bytes": [c_i64, c_i64, c_i64],
+    "seqvp_node_min_jd_extended": [c_vp, c_vp, c_i64, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_face_interior_minima": [c_vp, c_i64, c_vp, c_vp],
+    "seqvp_face_interior_minima_host": [c_vp, c_i64, c_vp],
     "seqvp_jd_root": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_prescreen": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_prescreen_flags": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
@@ -73,7 +77,8 @@ SIGNATURES = {
     "seqvp_axis_transform": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_sz, c_vp],
 }
 _RESTYPES = {"seqvp_version": ctypes.c_char_p, "seqvp_workspace_bytes": c_sz,
-             "seqvp_certificate_workspace_bytes": c_sz, "seqvp_preopt_workspace_bytes": c_sz}
+             "seqvp_certificate_workspace_bytes": c_sz, "seqvp_preopt_workspace_bytes": c_sz,
+             "seqvp_node_min_jd_extended_workspace_bytes": c_sz}
 
 
 class Certificate(ctypes.Structure):

@@ -8,6 +8,7 @@
 // stores), and one lane per node adds the eight contributions in vertex order a = 0..7 after the
 // node base -- no atomics, so every result is bit-reproducible.
 #include "q1_common.h"
+#include "jd_extension.h"
 #include "abi_util.h"
 
 namespace seqvp {
@@ -608,20 +609,15 @@ __device__ __forceinline__ double edge_quadratic_min(double f0, double f1, doubl
   return inside ? f0 - b * b / (4.0 * a) : __builtin_inf();
 }
 
-__global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __restrict__ u, const double* __restrict__ k,
-                                                            Dims d, double* __restrict__ minima) {
-  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (c >= d.ncell) return;
-  double ih[3];
-  load_ih(k, ih);
-  const int64_t b = cell_base_node(c, d);
+// The 27 closed-point Jacobians of cell c (index 9 iz + 3 iy + ix).
+__device__ __forceinline__ void cell_closed_jd(const double* __restrict__ u, const double ih[3], const Dims& d,
+                                               int64_t c, double J[27]) {
   Edges Eu;
   {
     double v[8][3];
-    load_cell(u, b, d, v);
+    load_cell(u, cell_base_node(c, d), d, v);
     make_edges(v, ih, Eu);
   }
-  double J[27];
 #pragma unroll
   for (int p = 0; p < 27; ++p) {
     double sp[3], V[3][3];
@@ -629,6 +625,21 @@ __global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __rest
     point_grad(Eu, sp, V);
     J[p] = jacobian(V);
   }
+}
+
+// m_c per cell. With EXT, a cell whose Bernstein hull reaches below min(m_c, activation) is appended to the
+// candidate list of the T69 extension (jd_affine.py:154-157; counters[0] counts them, list order is free).
+template <bool EXT>
+__global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __restrict__ u, const double* __restrict__ k,
+                                                            Dims d, double activation, double* __restrict__ minima,
+                                                            uint32_t* __restrict__ list,
+                                                            unsigned long long* __restrict__ counters) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  if (c >= d.ncell) return;
+  double ih[3];
+  load_ih(k, ih);
+  double J[27];
+  cell_closed_jd(u, ih, d, c, J);
   double m = J[0];
 #pragma unroll
   for (int p = 1; p < 27; ++p) m = fmin(m, J[p]);
@@ -646,6 +657,65 @@ __global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __rest
 #pragma unroll
     for (int ix = 0; ix < 3; ix += 2) m = fmin(m, edge_quadratic_min(J[3 * iy + ix], J[9 + 3 * iy + ix], J[18 + 3 * iy + ix]));
   minima[c] = m;
+  if (EXT && t69::bernstein_min_inplace(J) < fmin(m, activation)) list[atomicAdd(&counters[0], 1ull)] = (uint32_t)c;
+}
+
+// Order-preserving map of doubles to unsigned keys (for an integer atomic min).
+__device__ __forceinline__ unsigned long long ordered_key(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_ordered_key(unsigned long long key) {
+  return __longlong_as_double((long long)((key >> 63) ? (key & 0x7fffffffffffffffull) : ~key));
+}
+
+__global__ void jd_extension_init_kernel(unsigned long long* __restrict__ counters) {
+  if (threadIdx.x < 3) counters[threadIdx.x] = 0ull;
+  if (threadIdx.x == 3) counters[3] = ordered_key(__builtin_inf());
+}
+
+// m_c <- min(m_c, mu_c) on the candidate cells (jd_affine.py:176-188), one lane per candidate. counters[1..3]
+// collect the record: cells lowered by more than 0.1 %, cells lowered at all, the least mu.
+constexpr int EXT_BLOCK = 64;
+__global__ void __launch_bounds__(EXT_BLOCK) jd_extension_kernel(const double* __restrict__ u,
+                                                                 const double* __restrict__ k, Dims d,
+                                                                 const uint32_t* __restrict__ list,
+                                                                 unsigned long long* __restrict__ counters,
+                                                                 double* __restrict__ minima) {
+  const unsigned long long n = counters[0];
+  double ih[3];
+  load_ih(k, ih);
+  for (unsigned long long i = (unsigned long long)blockIdx.x * EXT_BLOCK + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * EXT_BLOCK) {
+    const int64_t c = list[i];
+    double J[27];
+    cell_closed_jd(u, ih, d, c, J);
+    const double mu = t69::face_interior_min(J);
+    const double m = minima[c];
+    if (mu < m * (1.0 - 1e-3)) atomicAdd(&counters[1], 1ull);
+    if (mu < m) atomicAdd(&counters[2], 1ull);
+    atomicMin(&counters[3], ordered_key(mu));
+    minima[c] = nan_min(m, mu);
+  }
+}
+
+__global__ void jd_extension_stats_kernel(const unsigned long long* __restrict__ counters, double* __restrict__ stats) {
+  if (threadIdx.x != 0) return;
+  stats[0] = (double)counters[0];
+  stats[1] = (double)counters[1];
+  stats[2] = (double)counters[2];
+  stats[3] = from_ordered_key(counters[3]);
+}
+
+// mu of each row of n closed27 rows (jd_affine.py:130-151).
+__global__ void __launch_bounds__(EXT_BLOCK) face_interior_rows_kernel(const double* __restrict__ J27, int64_t n,
+                                                                       double* __restrict__ mu) {
+  const int64_t i = (int64_t)blockIdx.x * EXT_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double J[27];
+#pragma unroll
+  for (int p = 0; p < 27; ++p) J[p] = J27[27 * i + p];
+  mu[i] = t69::face_interior_min(J);
 }
 
 __global__ void __launch_bounds__(BLOCK) node_min_kernel(const double* __restrict__ minima, Dims d,
@@ -1034,9 +1104,56 @@ extern "C" int seqvp_node_min_jd(const double* u, const double* k, int64_t nx, i
   const Dims d = make_dims(nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
   double* minima = (double*)ws;
-  hipLaunchKernelGGL(cell_min_jd_kernel, dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, k, d, minima);
+  hipLaunchKernelGGL(cell_min_jd_kernel<false>, dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, k, d, 0.0, minima,
+                     (uint32_t*)nullptr, (unsigned long long*)nullptr);
   hipLaunchKernelGGL(node_min_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, minima, d, node);
   return launch_status();
+}
+
+// Workspace of the extended measure: counters (4 x u64) | minima (ncell doubles) | candidate list (ncell u32).
+static size_t node_jd_ext_workspace(int64_t nx, int64_t ny, int64_t nz) {
+  const Dims d = make_dims(nx, ny, nz);
+  return 32 + (size_t)d.ncell * (sizeof(double) + sizeof(uint32_t));
+}
+
+extern "C" size_t seqvp_node_min_jd_extended_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+  return node_jd_ext_workspace(nx, ny, nz);
+}
+
+extern "C" int seqvp_node_min_jd_extended(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz,
+                                          double activation, double* node, double* stats, void* ws, size_t ws_bytes,
+                                          void* stream) {
+  if (!u || !k || !node || !grid_ok(nx, ny, nz) || !(activation > 0.0) || !(activation <= 1.0)) return SEQVP_EINVAL;
+  const Dims d = make_dims(nx, ny, nz);
+  if (d.ncell > 0xffffffffll) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < node_jd_ext_workspace(nx, ny, nz)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long* counters = (unsigned long long*)ws;
+  double* minima = (double*)((char*)ws + 32);
+  uint32_t* list = (uint32_t*)(minima + d.ncell);
+  hipLaunchKernelGGL(jd_extension_init_kernel, dim3(1), dim3(64), 0, s, counters);
+  hipLaunchKernelGGL(cell_min_jd_kernel<true>, dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, k, d, activation,
+                     minima, list, counters);
+  // one lane per candidate; the grid strides over the count the previous kernel left in counters[0]
+  const int64_t eblocks = cdiv(d.ncell, EXT_BLOCK) < 4096 ? cdiv(d.ncell, EXT_BLOCK) : 4096;
+  hipLaunchKernelGGL(jd_extension_kernel, dim3(eblocks), dim3(EXT_BLOCK), 0, s, u, k, d, list, counters, minima);
+  hipLaunchKernelGGL(node_min_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, minima, d, node);
+  if (stats) hipLaunchKernelGGL(jd_extension_stats_kernel, dim3(1), dim3(64), 0, s, counters, stats);
+  return launch_status();
+}
+
+extern "C" int seqvp_face_interior_minima(const double* J27, int64_t n, double* mu, void* stream) {
+  if (n < 0 || (n > 0 && (!J27 || !mu))) return SEQVP_EINVAL;
+  if (n == 0) return SEQVP_OK;
+  hipLaunchKernelGGL(face_interior_rows_kernel, dim3(cdiv(n, EXT_BLOCK)), dim3(EXT_BLOCK), 0, (hipStream_t)stream,
+                     J27, n, mu);
+  return launch_status();
+}
+
+extern "C" int seqvp_face_interior_minima_host(const double* J27, int64_t n, double* mu) {
+  if (n < 0 || (n > 0 && (!J27 || !mu))) return SEQVP_EINVAL;
+  for (int64_t i = 0; i < n; ++i) mu[i] = t69::face_interior_min(J27 + 27 * i);
+  return SEQVP_OK;
 }
 
 extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, double limit, int64_t nx, int64_t ny,

@@ -7,8 +7,9 @@ only scalars (energies, roots, dot products, counts) come back to the host -- th
 hybrid.py:262-286).
 
 Operations and the reference code they follow:
-  metric()     -- _pg_metric_inverse with the image_shift_ggn_equilibrated_jd_affine policy
-                  (poisson_gradient.py:400-576) without the T69 face/interior extension
+  metric()     -- _pg_metric_inverse with the image_shift_ggn_equilibrated_jd_affine policies
+                  (poisson_gradient.py:400-576): the T62 node measure, with the T69 face/interior extension
+                  on candidate cells when asked (jd_affine.py:154-225)
   gradient()   -- TorchLinearization.gradient (torch_ops.py:559-589) restricted to theta
   direction()  -- p = -S M^-1 S g (poisson_gradient.py:1241-1262, hybrid.py:262-286)
   jd_bound()   -- T66 global root on the direction (torch_ops.py:636-673)
@@ -117,6 +118,7 @@ class PairState:
         self._den = None
         self.energy = None          # Energies of the accepted state
         self.metric_info = {}
+        self._ext_stats = None
 
     # ---- small helpers -------------------------------------------------------------------------
     def _stream(self):
@@ -164,8 +166,9 @@ class PairState:
         return Energies(n[0], n[1], s[0], s[1], b[0])
 
     # ---- metric --------------------------------------------------------------------------------
-    def metric(self, *, shift_factor=1., jd_affine=True, activation=JD_AFFINE_ACTIVATION):
-        """P^-1 = S M^-1 S with M the shifted Q1 Poisson metric (poisson_gradient.py:269-293, 400-576)."""
+    def metric(self, *, shift_factor=1., jd_affine=True, activation=JD_AFFINE_ACTIVATION, extension=True):
+        """P^-1 = S M^-1 S with M the shifted Q1 Poisson metric (poisson_gradient.py:269-293, 400-576); S carries
+        the JD-affine factors of the node measure, with the T69 face/interior extension when `extension`."""
         q, w = self.q1, self.problem.weights
         q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
                      ggn=self.node_a)
@@ -185,13 +188,25 @@ class PairState:
         self.poisson = DevicePoisson(SpectralMetric(self.grid, a, bp, "q1"), self.device, ws=q.ws, out=self._den)
         inv_h2 = sum(1 / h ** 2 for h in self.grid.spacing)
         m = self.grid.omega * (a + bp * (8 / 9) * inv_h2)
-        node = q.node_min_jd(self.u) if jd_affine else None
+        stats = None
+        if jd_affine and extension:
+            if self._ext_stats is None:
+                self._ext_stats = torch.empty(4, dtype=torch.float64, device=self.device)
+            stats = self._ext_stats
+            node = q.node_min_jd(self.u, extension_activation=activation, stats=stats)
+        else:
+            node = q.node_min_jd(self.u) if jd_affine else None
         nz, ny, nx = self.shape
         _lib.call("seqvp_metric_scale", _lib.ptr(self.diag), _lib.ptr(node), float(m), float(activation),
                   JD_AFFINE_FLOOR, nx, ny, nz, _lib.ptr(self.scale), _lib.ptr(self.bad), self._stream())
         if int(self.bad.item()):
             raise FloatingPointError("GGN diagonal or node JD invalid")
         self.metric_info = dict(shift_alpha=alpha, a=a, b=bp, mass_diagonal=m)
+        if stats is not None:
+            cand, lowered, lowered_any, mu_min = stats.tolist()
+            self.metric_info["jd_affine"] = dict(activation=activation, extension=dict(
+                triggered="always", candidates=int(cand), lowered_cells=int(lowered),
+                lowered_cells_any=int(lowered_any), mu_min=mu_min))
         return self.metric_info
 
     # ---- gradient / direction / bound ------------------------------------------------------------

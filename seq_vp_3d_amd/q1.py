@@ -135,14 +135,37 @@ class FusedQ1:
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
-    def node_min_jd(self, u):
-        """T62 node JD measure (closed27 + exact edge minima), full grid shape, +inf off-grid."""
+    def node_min_jd(self, u, extension_activation=None, stats=None):
+        """T62 node JD measure (closed27 + exact edge minima), full grid shape, +inf off-grid.
+
+        With extension_activation (in (0, 1]) the T69 face/interior extension acts on the candidate cells
+        (jd_affine.py:154-225). stats: None, or a device tensor of 4 doubles that receives the number of
+        candidates, of cells lowered by more than 0.1 %, of cells lowered at all, and the least estimate."""
         self._check(u)
         out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
-        w, n = self._workspace(OP_NODEMIN)
-        _lib.call("seqvp_node_min_jd", _lib.ptr(u), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
+        if extension_activation is None:
+            w, n = self._workspace(OP_NODEMIN)
+            _lib.call("seqvp_node_min_jd", _lib.ptr(u), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
+                      _lib.stream_handle(self.device))
+            return out
+        if stats is not None and (stats.dtype != torch.float64 or stats.numel() < 4 or stats.device != self.device):
+            raise ValueError("stats must be a float64 device tensor of 4 entries")
+        buf, n = self.ws.get(_lib.raw("seqvp_node_min_jd_extended_workspace_bytes", *self._dims()))
+        _lib.call("seqvp_node_min_jd_extended", _lib.ptr(u), _lib.ptr(self.k), *self._dims(),
+                  float(extension_activation), _lib.ptr(out), _lib.ptr(stats), _lib.ptr(buf), n,
                   _lib.stream_handle(self.device))
         return out
+
+    @staticmethod
+    def face_interior_minima(J27):
+        """T69 estimate per row of a (n, 27) float64 device tensor (jd_affine.py:130-151)."""
+        J27 = J27.contiguous()
+        if J27.dtype != torch.float64 or J27.dim() != 2 or J27.shape[1] != 27:
+            raise ValueError("closed27 JD values must be a float64 (cells, 27) tensor")
+        mu = torch.empty(J27.shape[0], dtype=torch.float64, device=J27.device)
+        _lib.call("seqvp_face_interior_minima", _lib.ptr(J27), J27.shape[0], _lib.ptr(mu),
+                  _lib.stream_handle(J27.device))
+        return mu
 
 
 class FusedJDRoot:

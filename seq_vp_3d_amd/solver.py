@@ -2,7 +2,8 @@
 
 Restates the decisions of the reference's default route -- solve_poisson_gradient with
 ConservativePGConfig3D(resolution_exit=True, jd_step_global=True) and the
-image_shift_ggn_equilibrated_jd_affine metric (jd_curl/three_d/solvers/poisson_gradient.py:1090-1646), and
+image_shift_ggn_equilibrated_jd_affine_v3 metric (T62 node measure with the T69 face/interior extension always
+on; jd_curl/three_d/solvers/poisson_gradient.py:400-576, 1090-1646), and
 solve_sequential (jd_curl/three_d/sequential.py:151-689) -- on a PairState whose arrays never leave HBM.
 Only scalars cross PCIe. The step bound is the T66 global closed27 root (no historical witness memory:
 its prediction is the proposal itself). A trial is valid only when its closed27 samples are above delta and
@@ -17,6 +18,13 @@ import numpy as np
 import torch
 
 EPS = float(np.finfo(float).eps)
+
+# JD-affine metric policies (poisson_gradient.py:116-119, 480-490): the T62 node measure, the T69 measure with the
+# face/interior extension once the pair has met an undetermined certificate, and T73 with the extension always on
+# (the native default, usage.md:85).
+T62_POLICY = "image_shift_ggn_equilibrated_jd_affine_v1"
+T69_POLICY = "image_shift_ggn_equilibrated_jd_affine_v2"
+T73_POLICY = "image_shift_ggn_equilibrated_jd_affine_v3"
 
 
 @dataclass(frozen=True)
@@ -35,6 +43,7 @@ class PGConfig:
     jd_step_global: bool = True
     shift_factor: float = 1.
     jd_affine_activation: float = .25
+    metric_policy: str = T73_POLICY
     fp32_direction: bool = False
     fp32_screen: bool = False            # precision form: FP32 screening energies with FP64 rechecks
 
@@ -187,8 +196,13 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
         if accepted_steps >= config.max_iterations:
             return finish("max_iterations", "ACCEPTED_STEP_BUDGET")
         if not metric_ready:
+            if config.metric_policy not in (T62_POLICY, T69_POLICY, T73_POLICY):
+                raise ValueError(f"unknown metric policy {config.metric_policy!r}")
+            extend = config.metric_policy == T73_POLICY or (config.metric_policy == T69_POLICY
+                                                             and getattr(st, "uncertain_seen", 0) > 0)
             try:
-                st.metric(shift_factor=config.shift_factor, activation=config.jd_affine_activation)
+                st.metric(shift_factor=config.shift_factor, activation=config.jd_affine_activation,
+                          extension=extend)
             except FloatingPointError as exc:
                 return finish("nonfinite_evaluation", str(exc))
             metric_ready = True
@@ -247,6 +261,8 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                     accepted = certified > 0 and certified >= required
                     reason = ("ACCEPTED" if accepted else "ENERGY_RESOLUTION" if abs(actual) <= resolution
                               else "NO_STRICT_DECREASE" if actual < 0 else "ARMIJO_REJECTED")
+            if reason == "FEASIBILITY_UNCERTAIN":
+                st.uncertain_seen = getattr(st, "uncertain_seen", 0) + 1       # the T69 trigger (sticky)
             exit_info = None
             if (config.resolution_exit and not accepted and not reason.startswith("NONFINITE")
                     and global_attempts + 1 <= config.max_backtracks):

@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from oracle import certificate as ocert, jd_root as ojr, poisson as opo, q1 as Q, spline as osp
+from oracle import certificate as ocert, jd_affine as oja, jd_root as ojr, poisson as opo, q1 as Q, spline as osp
 from seq_vp_3d_amd.hotpath import Energies, TrialResult
 
 
@@ -56,7 +56,7 @@ class CpuPairState:
         return [float(np.dot(a.numpy().astype(np.longdouble), b.numpy())) for a, b in pairs]
 
     # metric / gradient / direction ---------------------------------------------------------------
-    def metric(self, *, shift_factor=1., jd_affine=True, activation=.25):
+    def metric(self, *, shift_factor=1., jd_affine=True, activation=.25, extension=True):
         m, w = self.m, self.weights
         base = m.ggn_diagonal_base(self.bgrad)
         diag = Q.restrict_free(m.ggn_diagonal_nodes(base, self.u), self.grid)
@@ -69,7 +69,9 @@ class CpuPairState:
         mm = self.grid.omega * (a + bp * (8 / 9) * inv_h2)
         scale = np.sqrt(mm) / np.sqrt(diag)
         if jd_affine:
-            node = m.node_min_jd(self.u)[1:-1, 1:-1, 1:-1]
+            node, self.extension_stats = oja.node_min_jd(self.u, self.grid, m.closed[3],
+                                                         extension_activation=activation if extension else None)
+            node = node[1:-1, 1:-1, 1:-1]
             r = np.minimum(1., np.maximum(node, 1e-12) / activation)
             scale = scale * np.broadcast_to(r, (3,) + r.shape).ravel()
         self.scale = scale

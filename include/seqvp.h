@@ -126,6 +126,19 @@ int seqvp_face_interior_minima(const double* J27, int64_t n, double* mu, void* s
  * jd_affine.py:163-172). */
 int seqvp_face_interior_minima_host(const double* J27, int64_t n, double* mu);
 
+/* JD witness nomination (replaces the tile loop of witness_screen.screen, witness_screen.py:55-76): for each tile
+ * of `tile` consecutive cells, out[26 t] = least closed27 J of the tile (nonfinite J counts as +inf),
+ * out[26 t + 1] = first flat index (cell - tile start) * 27 + point where it occurs, out[26 t + 2 ...] = the 24
+ * vertex values (vertex-major, x + 2y + 4z) of that cell. out holds 26 * ceil(ncell / tile) doubles. */
+int seqvp_closed27_tile_minima(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz, int64_t tile,
+                               double* out, void* stream);
+
+/* Cubic coefficients of det(A + s V) - delta at n reference points (x, y, z) of cells (replaces
+ * determinant_coefficients over JDStepMemory.predict's witness cells, jd_step.py:13-17, 204-222): A = I + grad u,
+ * V = grad v (node fields), delta = k[36]. out: (n, 4) ascending powers; NaN for a cell id outside the grid. */
+int seqvp_witness_coefficients(const double* u, const double* v, const double* k, int64_t nx, int64_t ny, int64_t nz,
+                               const int64_t* cells, const double* points, int64_t n, double* out, void* stream);
+
 /* T66 global closed27 first JD root (replaces FusedJDRoot.__call__, triton_jd_root.py:38-52;
  * global_jd_step.py:22-129). u: base field, v: direction field (both node fields), limit > 0.
  * out[0] = min root in s-units (inf if none), out[1] = number of points with c0 <= 0 (as double),
@@ -323,6 +336,11 @@ int seqvp_gather_vertices(const double* u, const int64_t* cells, int64_t n, int6
  * Returns 1 when every step of every cell was finite and ordered, 0 otherwise, SEQVP_EINVAL on bad arguments. */
 int seqvp_bernstein_arrays(const double* vertices, int64_t n, const double* spacing, double delta, double* p_lo,
                            double* p_hi, double* c_lo, double* c_hi, double* samples);
+
+/* HOST. Outward enclosure [out[0], out[1]] of J at the reference point (x, y, z) in [0,1]^3 of a cell with host
+ * vertices (8, 3) (replaces stationarity.point_jd_interval, stationarity.py:39-56, the CPU check of a nominated
+ * JD witness, witness_screen.py:79-88). out = NaN where the reference raises (a nonfinite or inverted step). */
+int seqvp_point_jd_interval_host(const double* vertices, const double* spacing, const double* point, double* out);
 
 /* HOST. Bernstein classification of one cell (jacobian_intervals + lagrange_to_bernstein + _classify with the
  * lowest-bound-first subdivision, feasibility.py:305-393). spacing = (hx, hy, hz). */

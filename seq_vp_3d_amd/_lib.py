@@ -31,6 +31,9 @@ SIGNATURES = {
     "seqvp_node_min_jd_extended_workspace_bytes": [c_i64, c_i64, c_i64],
     "seqvp_node_min_jd_extended": [c_vp, c_vp, c_i64, c_i64, c_i64, c_dbl, c_vp, c_vp, c_vp, c_sz, c_vp],
     "seqvp_face_interior_minima": [c_vp, c_i64, c_vp, c_vp],
+    "seqvp_closed27_tile_minima": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],
+    "seqvp_witness_coefficients": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp],
+    "seqvp_point_jd_interval_host": [c_vp, c_vp, c_vp, c_vp],
     "seqvp_face_interior_minima_host": [c_vp, c_i64, c_vp],
     "seqvp_jd_root": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_prescreen": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],

@@ -768,6 +768,23 @@ extern "C" int seqvp_bernstein_arrays(const double* vertices, int64_t n, const d
   return all_ok;
 }
 
+extern "C" int seqvp_point_jd_interval_host(const double* vertices, const double* spacing, const double* point,
+                                            double* out) {
+  if (!vertices || !spacing || !point || !out) return SEQVP_EINVAL;
+  for (int j = 0; j < 3; ++j)
+    if (!(spacing[j] > 0.0) || !(point[j] >= 0.0) || !(point[j] <= 1.0)) return SEQVP_EINVAL;
+  double v[8][3];
+  for (int a = 0; a < 8; ++a)
+    for (int r = 0; r < 3; ++r) v[a][r] = vertices[3 * a + r];
+  bool ok = true;
+  for (int a = 0; a < 8; ++a)
+    for (int r = 0; r < 3; ++r) ok = ok && seqvp::iv::is_finite(v[a][r]);
+  const seqvp::iv::I J = seqvp::iv::point_jd_interval(v, spacing, point, ok);
+  out[0] = ok ? J.lo : __builtin_nan("");
+  out[1] = ok ? J.hi : __builtin_nan("");
+  return SEQVP_OK;
+}
+
 extern "C" int seqvp_bernstein_cell(const double* vertices, const double* spacing, double delta, int max_depth,
                                     int64_t max_boxes, SeqvpCellCertificate* out) {
   if (!vertices || !spacing || !out || max_depth < 0 || max_boxes < 1) return SEQVP_EINVAL;

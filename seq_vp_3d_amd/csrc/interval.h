@@ -79,6 +79,15 @@ __host__ __device__ __forceinline__ I mul(I a, I b, bool& ok) {
   return outward(np_min(np_min(np_min(x, y), z), w), np_max(np_max(np_max(x, y), z), w), ok);
 }
 __host__ __device__ __forceinline__ I point(double v) { return I{v, v}; }
+// Division; a divisor interval containing zero is invalid (interval.py:87-94).
+__host__ __device__ __forceinline__ I div(I a, I b, bool& ok) {
+  if (b.lo <= 0.0 && b.hi >= 0.0) {
+    ok = false;
+    return a;
+  }
+  const double x = a.lo / b.lo, y = a.lo / b.hi, z = a.hi / b.lo, w = a.hi / b.hi;
+  return outward(np_min(np_min(np_min(x, y), z), w), np_max(np_max(np_max(x, y), z), w), ok);
+}
 
 // det of the interval matrix F[r][j] by the left-associated cofactor expansion along row 0
 // (compiled_enclosure.py:60-71; feasibility.py:213-216).
@@ -87,6 +96,38 @@ __host__ __device__ __forceinline__ I determinant(const I F[3][3], bool& ok) {
   const I b = sub(mul(F[1][2], F[2][0], ok), mul(F[1][0], F[2][2], ok), ok);
   const I c = sub(mul(F[1][0], F[2][1], ok), mul(F[1][1], F[2][0], ok), ok);
   return add(add(mul(F[0][0], a, ok), mul(F[0][1], b, ok), ok), mul(F[0][2], c, ok), ok);
+}
+
+// Enclosure of J = det(I + grad u) at the reference point t in [0,1]^3 of a cell with vertex values v
+// (stationarity.py:39-56): per derivative direction j, the sum over the four edges along j of
+// ((v_b - v_a) / h_j) * w, with the bilinear weight w = prod over the other axes k of (t_k or 1 - t_k) built
+// from the point one factor at a time; then F = that + identity and the row-0 cofactor determinant
+// F00 (F11 F22 - F12 F21) - F01 (F10 F22 - F12 F20) + F02 (F10 F21 - F11 F20). `ok` turns false where the
+// reference raises (a nonfinite or inverted step).
+__host__ __device__ inline I point_jd_interval(const double v[8][3], const double h[3], const double t[3], bool& ok) {
+  I F[3][3];
+  for (int j = 0; j < 3; ++j) {
+    I d[3] = {point(0.0), point(0.0), point(0.0)};
+    for (int a = 0; a < 8; ++a) {
+      if (a & (1 << j)) continue;
+      const int b = a | (1 << j);
+      I w = point(1.0);
+      for (int k = 0; k < 3; ++k) {
+        if (k == j) continue;
+        const I f = (a & (1 << k)) ? point(t[k]) : sub(point(1.0), point(t[k]), ok);
+        w = mul(w, f, ok);
+      }
+      for (int r = 0; r < 3; ++r) {
+        const I q = div(sub(point(v[b][r]), point(v[a][r]), ok), point(h[j]), ok);
+        d[r] = add(d[r], mul(q, w, ok), ok);
+      }
+    }
+    for (int r = 0; r < 3; ++r) F[r][j] = add(d[r], point(r == j ? 1.0 : 0.0), ok);
+  }
+  const I a = sub(mul(F[1][1], F[2][2], ok), mul(F[1][2], F[2][1], ok), ok);
+  const I b = sub(mul(F[1][0], F[2][2], ok), mul(F[1][2], F[2][0], ok), ok);
+  const I c = sub(mul(F[1][0], F[2][1], ok), mul(F[1][1], F[2][0], ok), ok);
+  return add(sub(mul(F[0][0], a, ok), mul(F[0][1], b, ok), ok), mul(F[0][2], c, ok), ok);
 }
 
 // The same operations on finite operands (FAST: next_*_finite and plain min/max). Once any step of a cell is

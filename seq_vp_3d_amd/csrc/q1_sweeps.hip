@@ -707,6 +707,123 @@ __global__ void jd_extension_stats_kernel(const unsigned long long* __restrict__
   stats[3] = from_ordered_key(counters[3]);
 }
 
+// ------------------------------------------------------------------------------------------------
+// JD witnesses (witness_screen.py:22-88; jd_step.py:13-17, 177-239).
+//
+// Per tile of `tile` consecutive cells: the least closed27 J (a nonfinite J counts as +inf), the first flat
+// index (cell - tile start) * 27 + p where it occurs, and the 24 vertex values of that cell, so the host can
+// check the nominated point with an interval enclosure. out: 26 doubles per tile.
+__global__ void __launch_bounds__(BLOCK) closed27_tile_minima_kernel(const double* __restrict__ u,
+                                                                     const double* __restrict__ k, Dims d,
+                                                                     int64_t tile, double* __restrict__ out) {
+  const int64_t t = blockIdx.x;
+  const int64_t start = t * tile, end = start + tile < d.ncell ? start + tile : d.ncell;
+  double ih[3];
+  load_ih(k, ih);
+  double best = __builtin_inf();
+  int64_t at = INT64_MAX;
+  for (int64_t c = start + threadIdx.x; c < end; c += BLOCK) {
+    double J[27];
+    cell_closed_jd(u, ih, d, c, J);
+#pragma unroll
+    for (int p = 0; p < 27; ++p) {
+      const double v = (J[p] - J[p] == 0.0) ? J[p] : __builtin_inf();
+      if (v < best) {
+        best = v;
+        at = (c - start) * 27 + p;
+      }
+    }
+  }
+  __shared__ double sv[BLOCK];
+  __shared__ int64_t si[BLOCK];
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = at;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double ov = sv[threadIdx.x + w];
+      const int64_t oi = si[threadIdx.x + w];
+      if (ov < sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = ov;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  double* o = out + t * 26;
+  if (threadIdx.x == 0) {
+    const int64_t first = si[0] == INT64_MAX ? 0 : si[0];
+    o[0] = sv[0];
+    o[1] = (double)first;
+  }
+  const int64_t cell = start + (si[0] == INT64_MAX ? 0 : si[0]) / 27;
+  if (threadIdx.x < 24) {
+    const int a = threadIdx.x / 3, r = threadIdx.x % 3;
+    o[2 + threadIdx.x] = u[3 * vertex_node(cell_base_node(cell, d), a, d) + r];
+  }
+}
+
+// det(A + s V) - delta = c0 + c1 s + c2 s^2 + c3 s^3 at reference points of cells (jd_step.py:13-17): A = I +
+// grad u and V = grad v at the point through the Q1 weights of the point (jd_step.py:204-222); c1 and c2 are the
+// cofactor contractions tr(adj(A) V) and tr(A adj(V)). out: 4 doubles per point.
+__global__ void __launch_bounds__(BLOCK) witness_coefficients_kernel(const double* __restrict__ u,
+                                                                     const double* __restrict__ v,
+                                                                     const double* __restrict__ k, Dims d,
+                                                                     const int64_t* __restrict__ cells,
+                                                                     const double* __restrict__ pts, int64_t n,
+                                                                     double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double ih[3];
+  load_ih(k, ih);
+  const double delta = k[K_DELTA];
+  if (cells[i] < 0 || cells[i] >= d.ncell) {   // never read outside the field
+    for (int q = 0; q < 4; ++q) out[4 * i + q] = __builtin_nan("");
+    return;
+  }
+  const int64_t b = cell_base_node(cells[i], d);
+  const double t[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+  double A[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}, V[3][3] = {};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t node = vertex_node(b, a, d);
+    double f[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) f[j] = (a >> j) & 1 ? t[j] : 1.0 - t[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double w = (((a >> j) & 1) ? ih[j] : -ih[j]) * f[other1(j)] * f[other2(j)];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        A[r][j] += w * u[3 * node + r];
+        V[r][j] += w * v[3 * node + r];
+      }
+    }
+  }
+  double cA[3][3], cV[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      cA[r][j] = A[r1][j1] * A[r2][j2] - A[r1][j2] * A[r2][j1];
+      cV[r][j] = V[r1][j1] * V[r2][j2] - V[r1][j2] * V[r2][j1];
+    }
+  double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      c1 += cA[r][j] * V[r][j];
+      c2 += A[r][j] * cV[r][j];
+    }
+  double* o = out + 4 * i;
+  o[0] = (A[0][0] * cA[0][0] + A[0][1] * cA[0][1] + A[0][2] * cA[0][2]) - delta;
+  o[1] = c1;
+  o[2] = c2;
+  o[3] = V[0][0] * cV[0][0] + V[0][1] * cV[0][1] + V[0][2] * cV[0][2];
+}
+
 // mu of each row of n closed27 rows (jd_affine.py:130-151).
 __global__ void __launch_bounds__(EXT_BLOCK) face_interior_rows_kernel(const double* __restrict__ J27, int64_t n,
                                                                        double* __restrict__ mu) {
@@ -1139,6 +1256,26 @@ extern "C" int seqvp_node_min_jd_extended(const double* u, const double* k, int6
   hipLaunchKernelGGL(jd_extension_kernel, dim3(eblocks), dim3(EXT_BLOCK), 0, s, u, k, d, list, counters, minima);
   hipLaunchKernelGGL(node_min_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, minima, d, node);
   if (stats) hipLaunchKernelGGL(jd_extension_stats_kernel, dim3(1), dim3(64), 0, s, counters, stats);
+  return launch_status();
+}
+
+extern "C" int seqvp_closed27_tile_minima(const double* u, const double* k, int64_t nx, int64_t ny, int64_t nz,
+                                          int64_t tile, double* out, void* stream) {
+  if (!u || !k || !out || !grid_ok(nx, ny, nz) || tile < 1) return SEQVP_EINVAL;
+  const Dims d = make_dims(nx, ny, nz);
+  hipLaunchKernelGGL(closed27_tile_minima_kernel, dim3(cdiv(d.ncell, tile)), dim3(BLOCK), 0, (hipStream_t)stream,
+                     u, k, d, tile, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_witness_coefficients(const double* u, const double* v, const double* k, int64_t nx, int64_t ny,
+                                          int64_t nz, const int64_t* cells, const double* points, int64_t n,
+                                          double* out, void* stream) {
+  if (!u || !v || !k || !grid_ok(nx, ny, nz) || n < 0 || (n > 0 && (!cells || !points || !out))) return SEQVP_EINVAL;
+  if (n == 0) return SEQVP_OK;
+  const Dims d = make_dims(nx, ny, nz);
+  hipLaunchKernelGGL(witness_coefficients_kernel, dim3(cdiv(n, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, u, v,
+                     k, d, cells, points, n, out);
   return launch_status();
 }
 

@@ -13,6 +13,7 @@ Operations and the reference code they follow:
   gradient()   -- TorchLinearization.gradient (torch_ops.py:559-589) restricted to theta
   direction()  -- p = -S M^-1 S g (poisson_gradient.py:1241-1262, hybrid.py:262-286)
   jd_bound()   -- T66 global root on the direction (torch_ops.py:636-673)
+  witness_coefficients() -- the cubics of the historical-witness predictor (jd_step.py:177-239)
   trial()      -- evaluate_trial: candidate map, whole-cell JD certificate (GPU edge hull + prescreen of every
                   cell, CPU interval Bernstein fallback of the undecided ones), sampling, energies vs base
                   (accelerated.py:1666-1730, 1574-1664; feasibility.py:571-769)
@@ -27,6 +28,7 @@ import torch
 from . import _lib
 from .poisson import DevicePoisson, SpectralMetric
 from .certificate import CertificateConfig, DeviceCertificate
+from .jd_step import nominate_witness
 from .q1 import FusedJDRoot, FusedQ1
 from .spline import DeviceSpline
 
@@ -67,6 +69,7 @@ class TrialResult:
     unproven_cells: int = 0                    # n_uncertain + n_infeasible of a failed certificate
     band: float | None = None                  # FP32 screening band of the differences (None: FP64 values)
     certificate: object = None                 # FeasibilityResult of the candidate (None: not reached)
+    witness: tuple | None = None               # confirmed (cell, reference point) with J <= delta
 
 
 class PairState:
@@ -226,6 +229,18 @@ class PairState:
         self.inject(self.p, self.v)
         return self.root(self.u, self.v, limit)
 
+    def witness_coefficients(self, cells, points):
+        """(n, 4) host array: det(A + s V) - delta at reference points of cells, A = I + grad u of the accepted
+        state, V = grad of the direction p (jd_step.py:13-17, 204-222)."""
+        self.inject(self.p, self.v)
+        c = torch.as_tensor(np.ascontiguousarray(cells, dtype=np.int64)).to(self.device)
+        pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64)).to(self.device)
+        out = torch.empty((len(cells), 4), dtype=torch.float64, device=self.device)
+        nz, ny, nx = self.shape
+        _lib.call("seqvp_witness_coefficients", _lib.ptr(self.u), _lib.ptr(self.v), _lib.ptr(self.q1.k), nx, ny, nz,
+                  _lib.ptr(c), _lib.ptr(pts), len(cells), _lib.ptr(out), self._stream())
+        return out.cpu().numpy()
+
     # ---- trial and promotion -----------------------------------------------------------------------
     def trial(self, tau, screen=False):
         """Candidate theta + tau p: certificate screen, sampling and energies against the accepted state.
@@ -256,14 +271,22 @@ class PairState:
         n, s, b, nbad = vals[0:4], vals[4:7], vals[7:11], vals[11]
         if nbad:
             return TrialResult(False, "INVALID_DOMAIN", None)
-        if not b[1] > q.delta or not np.isfinite(b[1]):
-            return TrialResult(False, "JACOBIAN_VIOLATION", None)
+        folded = not b[1] > q.delta or not np.isfinite(b[1])
+        if folded:
+            # a closed27 J at or below delta: the first tile's nominated point, confirmed by its enclosure, is a
+            # witness and rejects the candidate; unconfirmed, the whole-cell certificate decides
+            # (witness_screen.py:55-88; accelerated.py:1692-1729)
+            witness = nominate_witness(self.u_t, q.k, self.grid, q.delta, self.device)
+            if witness is not None:
+                return TrialResult(False, "JACOBIAN_VIOLATION", None, witness=witness)
         cert = self._certificate if self._certificate is not None else self.cert.finish(summary=vals[12:17])
         self._certificate = cert
         if not cert.certified:
             reason = "FEASIBILITY_UNCERTAIN" if cert.status == "uncertain" else cert.reason
             return TrialResult(False, reason, None, unproven_cells=cert.n_uncertain + cert.n_infeasible,
                                certificate=cert)
+        if folded:                      # certified, yet a float closed27 J sits at delta: no finite barrier
+            return TrialResult(False, "JACOBIAN_VIOLATION", None, certificate=cert)
         energy = Energies(n[0], n[1], s[0], s[1], b[0])
         change = q.barrier_weight * b[2]
         total = n[2] + n[3] + s[2] + change

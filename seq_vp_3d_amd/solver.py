@@ -5,17 +5,20 @@ ConservativePGConfig3D(resolution_exit=True, jd_step_global=True) and the
 image_shift_ggn_equilibrated_jd_affine_v3 metric (T62 node measure with the T69 face/interior extension always
 on; jd_curl/three_d/solvers/poisson_gradient.py:400-576, 1090-1646), and
 solve_sequential (jd_curl/three_d/sequential.py:151-689) -- on a PairState whose arrays never leave HBM.
-Only scalars cross PCIe. The step bound is the T66 global closed27 root (no historical witness memory:
-its prediction is the proposal itself). A trial is valid only when its closed27 samples are above delta and
-the whole-cell certificate proves J > delta on every cell (GPU edge hull and prescreen of every cell, the
-CPU interval Bernstein fallback for the cells they leave undecided; an uncertain certificate makes the trial
-FEASIBILITY_UNCERTAIN, a witness JACOBIAN_VIOLATION, as accelerated.py:1722-1729).
+Only scalars cross PCIe. The first trial step of an inner iteration is the historical-witness prediction
+(jd_step.py:177-239), capped by the T66 global closed27 root when jd_step_global is set. A trial is valid only
+when its closed27 samples are above delta and the whole-cell certificate proves J > delta on every cell (GPU
+edge hull and prescreen of every cell, the CPU interval Bernstein fallback for the cells they leave undecided;
+an uncertain certificate makes the trial FEASIBILITY_UNCERTAIN, a witness JACOBIAN_VIOLATION, as
+accelerated.py:1692-1729). Witnesses of rejected candidates feed the predictor.
 """
 from dataclasses import dataclass, field
 from time import perf_counter
 
 import numpy as np
 import torch
+
+from .jd_step import JDStepMemory
 
 EPS = float(np.finfo(float).eps)
 
@@ -161,8 +164,12 @@ def step_proposal(st, config, previous):
     return _protect(config, proposal), "bb2"
 
 
-def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=None):
-    """One frozen-reference inner solve on the accepted state of `st` (poisson_gradient.py:1090-1646)."""
+def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=None, memory=None):
+    """One frozen-reference inner solve on the accepted state of `st` (poisson_gradient.py:1090-1646). `memory` is
+    the JD witness memory of the level (a fresh one when None, poisson_gradient.py:1130-1136)."""
+    if memory is None:
+        memory = JDStepMemory()
+    memory.bind(st.grid)
     history = []
     accepted_steps, small_steps, trials = 0, 0, 0
     previous = None
@@ -199,7 +206,7 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             if config.metric_policy not in (T62_POLICY, T69_POLICY, T73_POLICY):
                 raise ValueError(f"unknown metric policy {config.metric_policy!r}")
             extend = config.metric_policy == T73_POLICY or (config.metric_policy == T69_POLICY
-                                                             and getattr(st, "uncertain_seen", 0) > 0)
+                                                             and memory.uncertain > 0)
             try:
                 st.metric(shift_factor=config.shift_factor, activation=config.jd_affine_activation,
                           extension=extend)
@@ -217,7 +224,9 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
         if bound <= 0:
             return finish("domain_blocked", "NO_POSITIVE_DOMAIN_STEP")
         tau = min(proposal, .9 * bound)
-        untruncated = tau == proposal
+        # JD step prediction: the historical witnesses, then (T66) the global closed27 root on the same limit
+        # (poisson_gradient.py:1356-1366, 708-735)
+        predicted, prediction = memory.predict(st.witness_coefficients, tau, config.jd_step_safety)
         root_info = None
         if config.jd_step_global:
             try:
@@ -231,7 +240,10 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 if root == 0:
                     root_info["fallback"] = "zero_root_below_resolution"
                 elif root is not None:
-                    tau = min(tau, config.jd_step_safety * root)
+                    predicted = min(predicted, config.jd_step_safety * root)
+            prediction["global"] = root_info
+        tau = predicted
+        untruncated = tau == proposal
         exit_floor = 15 * EPS * st.energy.total if config.resolution_exit else None
         global_attempts = 0
         accepted = False
@@ -244,6 +256,9 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 tr = screened_trial(st, tau, required)
             else:
                 tr = st.trial(tau)
+            if tr.witness is not None:
+                memory.witness(*tr.witness)
+            memory.certificate(tr.certificate)
             actual = resolution = certified = None
             if not tr.ok:
                 reason = tr.reason
@@ -261,8 +276,6 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                     accepted = certified > 0 and certified >= required
                     reason = ("ACCEPTED" if accepted else "ENERGY_RESOLUTION" if abs(actual) <= resolution
                               else "NO_STRICT_DECREASE" if actual < 0 else "ARMIJO_REJECTED")
-            if reason == "FEASIBILITY_UNCERTAIN":
-                st.uncertain_seen = getattr(st, "uncertain_seen", 0) + 1       # the T69 trigger (sticky)
             exit_info = None
             if (config.resolution_exit and not accepted and not reason.startswith("NONFINITE")
                     and global_attempts + 1 <= config.max_backtracks):
@@ -276,7 +289,7 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                        energy_resolution=resolution, certified_reduction=certified, energy_base=st.energy.total,
                        energy_trial=tr.energy.total if tr.energy is not None else None,
                        unproven_cells=tr.unproven_cells, certificate=certificate_record(tr.certificate),
-                       jd_root=root_info, resolution_exit=exit_info)
+                       jd_prediction=prediction, resolution_exit=exit_info)
             history.append(row)
             if observer is not None:
                 observer(row)
@@ -313,6 +326,8 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
     comparisons against the start, commit, self-reference refresh."""
     started = perf_counter()
     history = []
+    memory = JDStepMemory()             # the level's witness memory, shared by its inner solves (sequential.py:214-215)
+    memory.bind(st.grid)
     st.refresh_reference()
     GF, GD, GE = st.outer_norms()
     G0 = GF
@@ -338,7 +353,7 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
         st.outer_cache = None
         eta = config.eta0 if G0 == 0 else config.eta0 * np.sqrt(min(GF, G0) / G0)
         target = float(eta * GF)
-        inner = solve_poisson_gradient(st, target, inner_config, budget=budget)
+        inner = solve_poisson_gradient(st, target, inner_config, budget=budget, memory=memory)
         eligible = inner.status in ("converged_gradient", "max_iterations", "stagnated")
         accepted, reason = False, "INNER_FAILED_NOT_COMMITTED"
         total = fixed = None

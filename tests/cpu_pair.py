@@ -9,7 +9,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from oracle import certificate as ocert, jd_affine as oja, jd_root as ojr, poisson as opo, q1 as Q, spline as osp
+from oracle import certificate as ocert, jd_affine as oja, jd_root as ojr, jd_step as ojs, poisson as opo, q1 as Q, spline as osp
 from seq_vp_3d_amd.hotpath import Energies, TrialResult
 
 
@@ -95,6 +95,16 @@ class CpuPairState:
         v = Q.inject_free(self.p.numpy(), self.grid)
         return ojr.global_root(self.u, v, self.grid, self.D, self.m.delta, limit)
 
+    def witness_coefficients(self, cells, points):
+        v = Q.inject_free(self.p.numpy(), self.grid)
+        out = []
+        for cell in sorted(set(int(c) for c in cells)):
+            pts = np.asarray(points)[np.asarray(cells) == cell]
+            A = ojs.point_gradients(self.u, self.grid, cell, pts)
+            V = ojs.point_gradients(v, self.grid, cell, pts) - np.eye(3)
+            out.append(ojs.determinant_coefficients(A, V, self.m.delta))
+        return np.concatenate(out)
+
     def step_bound(self):
         v = self.p.numpy().reshape((3,) + self.grid.interior_shape)
         pts = (self.grid.identity() + self.u)[1:-1, 1:-1, 1:-1]
@@ -131,13 +141,18 @@ class CpuPairState:
         if np.any(pts < lo) or np.any(pts > hi) or np.any(inner <= lo) or np.any(inner >= hi):
             return TrialResult(False, "INVALID_DOMAIN", None)
         J = Q.geometry(u, self.grid, self.D)[2]
-        if not J.min() > self.m.delta:
-            return TrialResult(False, "JACOBIAN_VIOLATION", None)
+        folded = not J.min() > self.m.delta
+        if folded:
+            witness = ojs.nominate_witness(u, self.grid, self.D, self.m.delta)
+            if witness is not None:
+                return TrialResult(False, "JACOBIAN_VIOLATION", None, witness=witness)
         cert = ocert.certify_map(u, self.grid.shape, self.grid.spacing, self.grid.axes, self.m.delta, D=self.D)
         if cert["status"] != "certified_feasible":
             reason = "FEASIBILITY_UNCERTAIN" if cert["status"] == "uncertain" else cert["reason"]
             return TrialResult(False, reason, None, unproven_cells=cert["n_uncertain"] + cert["n_infeasible"],
                                certificate=cert)
+        if folded:
+            return TrialResult(False, "JACOBIAN_VIOLATION", None, certificate=cert)
         value, grad = osp.sample(self.spline, pts)
         self._trial = (u, value, grad)
         e = self.evaluate_energy(u, value)

@@ -69,3 +69,50 @@ def test_fp32_screen_keeps_the_fp64_decisions(gpu_device):
     assert torch.equal(st64.theta, st32.theta)
     rec = st32.screen_record
     assert rec["uses"] > 0 and rec["checks"] >= 1 and not rec["disabled"] and rec["max_rel"] <= 1.
+
+
+@pytest.mark.parametrize("jd_step_global", [False, True])
+def test_witness_memory_and_prediction_match_cpu_twin(gpu_device, jd_step_global):
+    """A first step far too long folds the map: the device nominates the same confirmed witness, the memory
+    predicts the same steps, and every attempt makes the same decision with the same tau."""
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.jd_step import JDStepMemory
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.solver import PGConfig, solve_poisson_gradient
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print(f"case witness memory jd_step_global={jd_step_global}", flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    spec = PairSpec((14, 15, 16), 6, (2., 4.), 3, .6, .01, 3)
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    config = PGConfig(max_iterations=5, jd_step_global=jd_step_global, initial_step=10.)
+    cpu = CpuPairState(spec.shape, (1., 1., 1.), w, pair.fixed.numpy(), pair.moving.numpy())
+    theta0 = Q.restrict_free(.5 * pair.displacement.numpy(), cpu.grid)
+    cpu.set_state(theta0)
+    gpu = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+    gpu.set_state(theta=torch.tensor(theta0, device=gpu_device))
+    runs = []
+    for st in (cpu, gpu):
+        st.refresh_reference()
+        rows, mem = [], JDStepMemory()
+        res = solve_poisson_gradient(st, 0., config, observer=rows.append, memory=mem)
+        runs.append((res, rows, mem))
+    (rc, rows_c, mem_c), (rg, rows_g, mem_g) = runs
+    assert rc.status == rg.status and rc.accepted_steps == rg.accepted_steps
+    assert len(rows_c) == len(rows_g)
+    for a, b in zip(rows_c, rows_g):
+        assert a["reason"] == b["reason"]
+        assert abs(a["tau"] - b["tau"]) <= 1e-12 * a["tau"]
+        pa, pb = a["jd_prediction"], b["jd_prediction"]
+        assert pa["witnesses"] == pb["witnesses"] and pa["points"] == pb["points"]
+        assert (pa["root"] is None) == (pb["root"] is None)
+        if pa["root"] is not None:
+            assert abs(pa["root"] - pb["root"]) <= 1e-10 * pa["root"]
+    if not jd_step_global:
+        assert any(r["reason"] == "JACOBIAN_VIOLATION" for r in rows_g)
+        assert len(mem_g.points) >= 1
+    assert list(mem_c.points) == list(mem_g.points)

@@ -50,8 +50,9 @@ def test_inner_attempt_rows_follow_the_acceptance_rule(pair):
         if r["accepted"]:
             assert r["certified_reduction"] > 0 and r["certified_reduction"] >= -1e-4 * r["tau"] * r["gradient_dot_direction"]
         assert r["tau"] <= .9 * r["domain_step_bound"] + 1e-300
-        if r["jd_root"] and r["jd_root"]["root"] is not None:
-            assert r["tau"] <= .9 * r["jd_root"]["root"] * (1 + 1e-15) or r["tau"] < r["jd_root"]["limit"]
+        root = r["jd_prediction"].get("global")
+        if root and root["root"] is not None:
+            assert r["tau"] <= .9 * root["root"] * (1 + 1e-15) or r["tau"] < root["limit"]
     assert rows[0]["step_rule"] == "initial" and all(r["step_rule"] != "initial" for r in rows[1:])
 
 

@@ -41,13 +41,17 @@ template <typename T> __device__ __forceinline__ int acc_row(int lg, int r);
 template <> __device__ __forceinline__ int acc_row<double>(int lg, int r) { return lg + 4 * r; }
 template <> __device__ __forceinline__ int acc_row<float>(int lg, int r) { return 4 * lg + r; }
 
-// sine matrix of length n into S (n*n), row-major, exact angle reduction m = (j+1)(k+1) mod 2(n+1),
-// evaluated in binary64 and rounded once to T
+// sine matrix of length n into S (ld x ld, ld >= n, zero outside n x n), row-major, exact angle reduction
+// m = (j+1)(k+1) mod 2(n+1), evaluated in binary64 and rounded once to T
 template <typename T>
-__global__ void sine_matrix_kernel(T* __restrict__ S, int64_t n) {
+__global__ void sine_matrix_kernel(T* __restrict__ S, int64_t n, int64_t ld) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n * n) return;
-  const int64_t k = i / n, j = i % n;
+  if (i >= ld * ld) return;
+  const int64_t k = i / ld, j = i % ld;
+  if (k >= n || j >= n) {
+    S[i] = (T)0;
+    return;
+  }
   const int64_t period = 2 * (n + 1);
   const int64_t m = ((j + 1) * (k + 1)) % period;
   S[i] = (T)(sqrt(2.0 / (double)(n + 1)) * sinpi((double)m / (double)(n + 1)));
@@ -207,6 +211,247 @@ __global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, 
       }
 }
 
+// ---- folded full-width passes (n <= 256) -------------------------------------------------------------------
+// The DST-I matrix satisfies S[n-1-j][k] = (-1)^k S[j][k]. Pairing j with n-1-j, an output k is
+//   even k: sum_{j < h'} U[j] S[j][k],  U[j] = x[j] + x[n-1-j] (j < h), U[h] = x[h] when n is odd,
+//   odd k:  sum_{j < h}  V[j] S[j][k],  V[j] = x[j] - x[n-1-j],
+// with h = floor(n/2), h' = ceil(n/2): two half-length contractions, half the MFMA work of the plain one.
+//
+// One workgroup of 2 NH waves (NH = ceil(h'/64)) owns a 64-wide tile of the independent index and every output
+// of it: waves 0..NH-1 compute the even outputs 2c, waves NH..2NH-1 the odd outputs 2c+1, c in [64w', 64w'+64),
+// as 4x4 v_mfma tiles, so the input is read from HBM once. U and V are formed while a k-slab (FK = 16 folded
+// indices) is written to LDS, double-buffered: the next slab's global loads are in flight in registers while
+// the waves run the current slab's 64 MFMAs each, one barrier per slab. Every load is unconditional from a
+// clamped in-bounds address (zero padding by select), the sine fragments of a slab are issued before its first
+// MFMA, and the folded sine matrix Sf (KP x LDF, zero-padded: Sf[j][c] = S[j][2c] for c < 64 NH, S[j][2c'+1]
+// with c' = c - 64 NH after) needs no bounds test, so no wait sits between a load and the MFMAs it overlaps.
+constexpr int FK = 16;
+
+__device__ __forceinline__ int64_t clamp_index(int64_t i, int64_t n) { return i < n ? i : n - 1; }
+
+// folded sine matrix of length n: KP rows (folded index j), LDF = 128 NH columns (even then odd outputs)
+template <typename T>
+__global__ void fold_sine_kernel(T* __restrict__ Sf, int64_t n, int64_t KP, int64_t LDF) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= KP * LDF) return;
+  const int64_t j = i / LDF, c = i % LDF, half = LDF / 2;
+  const int64_t hp = (n + 1) / 2, h = n / 2;
+  const bool odd = c >= half;
+  const int64_t cc = odd ? c - half : c, k = odd ? 2 * cc + 1 : 2 * cc;
+  if (odd ? (j >= h || cc >= h) : (j >= hp || cc >= hp)) {
+    Sf[i] = (T)0;
+    return;
+  }
+  const int64_t m = ((j + 1) * (k + 1)) % (2 * (n + 1));
+  Sf[i] = (T)(sqrt(2.0 / (double)(n + 1)) * sinpi((double)m / (double)(n + 1)));
+}
+
+// x axis: Out[o][k] = sum_j X[o][j] S[j][k] for 64 rows o per workgroup.
+template <typename T, typename TIN, typename TOUT, int PRE, int POST, int NH>
+__global__ void __launch_bounds__(128 * NH, 2) dst_rows_fold_kernel(const TIN* __restrict__ X, const double* __restrict__ pre,
+                                                                    const T* __restrict__ Sf, int64_t LDF, int64_t O,
+                                                                    int64_t n, const double* __restrict__ post,
+                                                                    double sign, TOUT* __restrict__ Y) {
+  static_assert(POST != POST_DIV, "the spectral division is fused into a strided pass");
+  typedef typename Acc<T>::type AT;
+  constexpr int NT = 128 * NH, PER = (TM * FK + NT - 1) / NT;
+  __shared__ T As[2][2][FK][TM + 1];   // As[buffer][U/V][k][m]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int side = wave >= NH ? 1 : 0;             // 0 even outputs (U), 1 odd outputs (V)
+  const int64_t m0 = xcd_block(blockIdx.x, gridDim.x) * TM;
+  const int c0 = 64 * (wave - side * NH);
+  const int64_t h = n / 2, hp = (n + 1) / 2;
+  AT acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  TIN xa[PER], xb[PER];
+  double pa[PER], pb[PER];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = t * NT + threadIdx.x;
+      const int ec = e < TM * FK ? e : 0;
+      const int64_t row = clamp_index(m0 + ec / FK, O) * n;
+      const int64_t j = clamp_index(k0 + ec % FK, hp);
+      xa[t] = X[row + j];
+      xb[t] = X[row + n - 1 - j];
+      if (PRE == PRE_SCALE) {
+        pa[t] = pre[row + j];
+        pb[t] = pre[row + n - 1 - j];
+      }
+    }
+  };
+  fetch(0);
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < hp; k0 += FK) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = t * NT + threadIdx.x;
+      if (e < TM * FK) {
+        const int r = e / FK, kk = e % FK;
+        const int64_t j = k0 + kk;
+        const T a = PRE == PRE_SCALE ? (T)(pa[t] * (double)xa[t]) : (T)xa[t];
+        const T b = PRE == PRE_SCALE ? (T)(pb[t] * (double)xb[t]) : (T)xb[t];
+        const bool live = m0 + r < O;
+        As[buf][0][kk][r] = (live && j < h) ? a + b : ((live && j < hp) ? a : (T)0);
+        As[buf][1][kk][r] = (live && j < h) ? a - b : (T)0;
+      }
+    }
+    __syncthreads();
+    // sine fragments first, then the next slab: the memory counter retires loads in order, so the MFMAs wait
+    // only for the (L2) fragments while the slab's HBM loads stay in flight
+    T bf[FK / 4][4];
+#pragma unroll
+    for (int q = 0; q < FK / 4; ++q)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[q][b] = Sf[(k0 + 4 * q + lk) * LDF + 64 * wave + 16 * b + li];
+    fetch(k0 + FK);   // unconditional (clamped past the end) so the wait counts stay static
+    __builtin_amdgcn_sched_barrier(0);   // keep the slab's loads ahead of the MFMAs they overlap
+#pragma unroll
+    for (int q = 0; q < FK / 4; ++q) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const T af = As[buf][side][4 * q + lk][16 * a + li];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mfma(af, bf[q][b], acc[a][b]);
+      }
+    }
+    buf ^= 1;
+  }
+  // epilogue, one 16-row band at a time: its 16 scale values are loaded (clamped, unconditionally) before any
+  // store, so the stores do not each wait for a load
+  const int64_t nout = side ? h : hp;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double pv[4][4];
+    if (POST == POST_SCALE) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r), c = c0 + 16 * b + li;
+          pv[b][r] = post ? post[clamp_index(gm, O) * n + clamp_index(2 * c + side, n)] : 1.0;
+        }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r), c = c0 + 16 * b + li;
+        if (gm < O && c < nout) {
+          const int64_t idx = gm * n + 2 * c + side;
+          if (POST == POST_SCALE) {
+            const double v = (double)acc[a][b][r];
+            Y[idx] = (TOUT)(sign * (post ? pv[b][r] * v : v));
+          } else {
+            Y[idx] = (TOUT)acc[a][b][r];
+          }
+        }
+      }
+  }
+}
+
+// strided axes: Out_o[k][i] = sum_j S[k][j] X_o[j][i] for 64 inner columns i per workgroup (S symmetric, so the
+// A fragment S[k][j] is Sf's entry for folded row j and output k).
+template <typename T, int POST, int NH>
+__global__ void __launch_bounds__(128 * NH, 2) dst_cols_fold_kernel(const T* __restrict__ X, const T* __restrict__ Sf,
+                                                                    int64_t LDF, int64_t n, int64_t I,
+                                                                    const double* __restrict__ den, T* __restrict__ Y) {
+  typedef typename Acc<T>::type AT;
+  constexpr int NT = 128 * NH, PER = (FK * TN + NT - 1) / NT;
+  __shared__ T Bs[2][2][FK][TN + 1];   // Bs[buffer][U/V][k][i]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int side = wave >= NH ? 1 : 0;
+  const int64_t nib = (I + TN - 1) / TN;
+  const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t o = tile / nib, n0 = (tile % nib) * TN;
+  const T* Xo = X + o * n * I;
+  T* Yo = Y + o * n * I;
+  const int r0 = 64 * (wave - side * NH);
+  const int64_t h = n / 2, hp = (n + 1) / 2;
+  AT acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  const int li = lane & 15, lk = lane >> 4;
+  T xa[PER], xb[PER];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = t * NT + threadIdx.x;
+      const int ec = e < FK * TN ? e : 0;
+      const int64_t j = clamp_index(k0 + ec / TN, hp), col = clamp_index(n0 + ec % TN, I);
+      xa[t] = Xo[j * I + col];
+      xb[t] = Xo[(n - 1 - j) * I + col];
+    }
+  };
+  fetch(0);
+  int buf = 0;
+  for (int64_t k0 = 0; k0 < hp; k0 += FK) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int e = t * NT + threadIdx.x;
+      if (e < FK * TN) {
+        const int kk = e / TN, c = e % TN;
+        const int64_t j = k0 + kk;
+        const bool live = n0 + c < I;
+        Bs[buf][0][kk][c] = (live && j < h) ? xa[t] + xb[t] : ((live && j < hp) ? xa[t] : (T)0);
+        Bs[buf][1][kk][c] = (live && j < h) ? xa[t] - xb[t] : (T)0;
+      }
+    }
+    __syncthreads();
+    T af[FK / 4][4];
+#pragma unroll
+    for (int q = 0; q < FK / 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) af[q][a] = Sf[(k0 + 4 * q + lk) * LDF + 64 * wave + 16 * a + li];
+    fetch(k0 + FK);   // unconditional (clamped past the end) so the wait counts stay static
+    __builtin_amdgcn_sched_barrier(0);   // keep the slab's loads ahead of the MFMAs they overlap
+#pragma unroll
+    for (int q = 0; q < FK / 4; ++q) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const T bf = Bs[buf][side][4 * q + lk][16 * b + li];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a][b] = mfma(af[q][a], bf, acc[a][b]);
+      }
+    }
+    buf ^= 1;
+  }
+  // epilogue per 16-row band; with POST_DIV the band's 16 denominators are loaded (clamped) before its stores
+  const int64_t nout = side ? h : hp;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T dv[4][4];
+    if (POST == POST_DIV) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t c = r0 + 16 * a + acc_row<T>(lk, r), gi = n0 + 16 * b + li;
+          dv[b][r] = (T)den[clamp_index(2 * c + side, n) * I + clamp_index(gi, I)];
+        }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t c = r0 + 16 * a + acc_row<T>(lk, r), gi = n0 + 16 * b + li;
+        if (c < nout && gi < I) {
+          T v = acc[a][b][r];
+          if (POST == POST_DIV) v = v / dv[b][r];
+          Yo[(2 * c + side) * I + gi] = v;
+        }
+      }
+  }
+}
+
 // Layout probe: D = A(16x4) * B(4x16) with one MFMA of type T, A[i][k] = a[i*4+k],
 // B[k][j] = b[k*16+j]; D[i][j] is written to d[i*16+j] through the lane mapping the kernels above
 // assume (operands and acc_row). The tests compare d with A*B for both instruction types.
@@ -250,8 +495,17 @@ __global__ void __launch_bounds__(256) poisson_denominator_kernel(const double* 
 
 using namespace seqvp;
 
+static inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+// Folded sine matrix of length n <= 256: NH = ceil(ceil(n/2)/64) wave pairs, KP = ceil(n/2) rounded up to FK rows,
+// LDF = 128 NH columns (even outputs, then odd outputs).
+static inline int64_t fold_nh(int64_t n) { return cdiv64(cdiv64(n, 2), 64); }
+static inline int64_t fold_kp(int64_t n) { return FK * cdiv64(cdiv64(n, 2), FK); }
+static inline int64_t fold_ld(int64_t n) { return 128 * fold_nh(n); }
+// Words of the stored sine matrix of length n (folded for n <= 256, plain otherwise).
+static inline int64_t sine_words(int64_t n) { return n <= 256 ? fold_kp(n) * fold_ld(n) : n * n; }
+
 size_t seqvp_dst_workspace_bytes(int64_t mx, int64_t my, int64_t mz, int64_t ncomp) {
-  return (size_t)(ncomp * mx * my * mz + mx * mx + my * my + mz * mz) * sizeof(double);
+  return (size_t)(ncomp * mx * my * mz + sine_words(mx) + sine_words(my) + sine_words(mz)) * sizeof(double);
 }
 
 extern "C" int seqvp_mfma_probe(const double* a, const double* b, int fp32, double* d, void* stream) {
@@ -263,7 +517,48 @@ extern "C" int seqvp_mfma_probe(const double* a, const double* b, int fp32, doub
   return launch_status();
 }
 
-static inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// One pass along x (rows of length n): the folded kernel for n <= 256 (S = the folded matrix, LDF columns), the
+// 64x64 tiling with the plain n x n matrix otherwise.
+template <typename T, typename TIN, typename TOUT, int PRE, int POST>
+static void rows_pass(const TIN* X, const double* pre, const T* S, int64_t O, int64_t n, const double* post,
+                      double sign, TOUT* Y, hipStream_t s) {
+  const dim3 g(cdiv64(O, TM));
+  const int64_t ldf = fold_ld(n);
+  if (n > 256)
+    hipLaunchKernelGGL((dst_rows_kernel<T, TIN, TOUT, PRE, POST>), dim3(cdiv64(O, TM) * cdiv64(n, TN)), dim3(256), 0, s,
+                       X, pre, S, O, n, post, sign, Y);
+  else if (fold_nh(n) == 1)
+    hipLaunchKernelGGL((dst_rows_fold_kernel<T, TIN, TOUT, PRE, POST, 1>), g, dim3(128), 0, s, X, pre, S, ldf, O, n, post, sign, Y);
+  else
+    hipLaunchKernelGGL((dst_rows_fold_kernel<T, TIN, TOUT, PRE, POST, 2>), g, dim3(256), 0, s, X, pre, S, ldf, O, n, post, sign, Y);
+}
+
+// One strided pass of length n over `batch` blocks of inner size I.
+template <typename T, int POST>
+static void cols_pass(const T* X, const T* S, int64_t n, int64_t I, int64_t batch, const double* den, T* Y,
+                      hipStream_t s) {
+  const dim3 g(cdiv64(I, TN) * batch);
+  const int64_t ldf = fold_ld(n);
+  if (n > 256)
+    hipLaunchKernelGGL((dst_cols_kernel<T, POST>), dim3(cdiv64(I, TN) * cdiv64(n, TM) * batch), dim3(256), 0, s, X, S,
+                       n, I, den, Y);
+  else if (fold_nh(n) == 1)
+    hipLaunchKernelGGL((dst_cols_fold_kernel<T, POST, 1>), g, dim3(128), 0, s, X, S, ldf, n, I, den, Y);
+  else
+    hipLaunchKernelGGL((dst_cols_fold_kernel<T, POST, 2>), g, dim3(256), 0, s, X, S, ldf, n, I, den, Y);
+}
+
+// The sine matrix of length n into S: folded (fold_kp(n) x fold_ld(n)) for n <= 256, plain n x n otherwise.
+template <typename T>
+static void sine_matrix(T* S, int64_t n, hipStream_t s) {
+  if (n > 256) {
+    hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(n * n, 256)), dim3(256), 0, s, S, n, n);
+  } else {
+    const int64_t kp = fold_kp(n), ld = fold_ld(n);
+    hipLaunchKernelGGL((fold_sine_kernel<T>), dim3(cdiv64(kp * ld, 256)), dim3(256), 0, s, S, n, kp, ld);
+  }
+}
 
 // T = compute/intermediate type. Intermediates ping-pong between the workspace and `out` (which has
 // room for ncomp*mx*my*mz doubles, so for T = float it holds them too).
@@ -278,35 +573,29 @@ static int dst_poisson(const double* rhs, const double* den, const double* scale
   const int64_t total = ncomp * mx * my * mz;
   T* tmp = (T*)ws;
   T* Sx = tmp + total;
-  T* Sy = Sx + mx * mx;
-  T* Sz = Sy + my * my;
+  T* Sy = Sx + sine_words(mx);
+  T* Sz = Sy + sine_words(my);
   T* alt = (T*)out;
-  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(mx * mx, 256)), dim3(256), 0, s, Sx, mx);
-  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(my * my, 256)), dim3(256), 0, s, Sy, my);
-  hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(mz * mz, 256)), dim3(256), 0, s, Sz, mz);
+  sine_matrix<T>(Sx, mx, s);
+  sine_matrix<T>(Sy, my, s);
+  sine_matrix<T>(Sz, mz, s);
   const int64_t O = ncomp * mz * my;
   const double sign = negate ? -1.0 : 1.0;
-  const dim3 grow(cdiv64(O, TM) * cdiv64(mx, TN));
   // forward x (pre-scale): rhs -> tmp
   if (scale)
-    hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_SCALE, POST_NONE>), grow, dim3(256), 0, s, rhs, scale, Sx, O,
-                       mx, nullptr, 1.0, tmp);
+    rows_pass<T, double, T, PRE_SCALE, POST_NONE>(rhs, scale, Sx, O, mx, nullptr, 1.0, tmp, s);
   else
-    hipLaunchKernelGGL((dst_rows_kernel<T, double, T, PRE_NONE, POST_NONE>), grow, dim3(256), 0, s, rhs, nullptr, Sx, O,
-                       mx, nullptr, 1.0, tmp);
+    rows_pass<T, double, T, PRE_NONE, POST_NONE>(rhs, nullptr, Sx, O, mx, nullptr, 1.0, tmp, s);
   // forward y: tmp -> alt (batch ncomp*mz, inner mx)
-  const dim3 gy(cdiv64(mx, TN) * cdiv64(my, TM) * ncomp * mz);
-  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gy, dim3(256), 0, s, tmp, Sy, my, mx, nullptr, alt);
+  cols_pass<T, POST_NONE>(tmp, Sy, my, mx, ncomp * mz, nullptr, alt, s);
   // forward z + division by the spectrum: alt -> tmp (batch ncomp, inner my*mx)
-  const dim3 gz(cdiv64(my * mx, TN) * cdiv64(mz, TM) * ncomp);
-  hipLaunchKernelGGL((dst_cols_kernel<T, POST_DIV>), gz, dim3(256), 0, s, alt, Sz, mz, my * mx, den, tmp);
+  cols_pass<T, POST_DIV>(alt, Sz, mz, my * mx, ncomp, den, tmp, s);
   // inverse z: tmp -> alt
-  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gz, dim3(256), 0, s, tmp, Sz, mz, my * mx, nullptr, alt);
+  cols_pass<T, POST_NONE>(tmp, Sz, mz, my * mx, ncomp, nullptr, alt, s);
   // inverse y: alt -> tmp
-  hipLaunchKernelGGL((dst_cols_kernel<T, POST_NONE>), gy, dim3(256), 0, s, alt, Sy, my, mx, nullptr, tmp);
+  cols_pass<T, POST_NONE>(alt, Sy, my, mx, ncomp * mz, nullptr, tmp, s);
   // inverse x + sign*(s*x) in binary64: tmp -> out
-  hipLaunchKernelGGL((dst_rows_kernel<T, T, double, PRE_NONE, POST_SCALE>), grow, dim3(256), 0, s, tmp, nullptr, Sx, O,
-                     mx, scale, sign, out);
+  rows_pass<T, T, double, PRE_NONE, POST_SCALE>(tmp, nullptr, Sx, O, mx, scale, sign, out, s);
   return launch_status();
 }
 

@@ -211,7 +211,9 @@ def test_mfma_layout_probe(gpu_device, fp32):
     np.testing.assert_allclose(d.cpu().numpy().reshape(16, 16), a @ b, rtol=tol, atol=tol)
 
 
-@pytest.mark.parametrize("shape", [(9, 10, 11), (20, 23, 37)])
+# interior lengths: odd and even, one wave pair (<= 128) and two (129..256) of the folded passes, and the plain
+# 64x64 tiling beyond 256 (x = 258)
+@pytest.mark.parametrize("shape", [(9, 10, 11), (20, 23, 37), (135, 5, 140), (4, 200, 7), (5, 6, 260)])
 def test_dst_poisson(gpu_device, shape):
     from seq_vp_3d_amd.poisson import DevicePoisson, build_poisson_inverse
     from seq_vp_3d_amd.problem import make_grid_3d

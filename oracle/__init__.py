@@ -25,4 +25,14 @@ reference file:line it follows):
 * `spline`    -- not-a-knot cubic tensor spline. Pinned by exact cubic
                  reproduction and node interpolation.
 * `synthetic` -- the synthetic pair generator (blobs + sinusoidal modes).
+* `certificate` -- the whole-cell interval certificate (edge hull, prescreen,
+                 Bernstein fallback, map records). Pinned by exact rationals,
+                 closed forms and the T05 folding cell.
+* `preopt`    -- the FP32 pre-optimiser's objective, Adam/cap step and pyramid.
+                 Pinned by finite differences and SciPy's own routines.
+* `jd_affine` -- the T62 node JD measure and its T69 face/interior extension.
+                 Pinned by quadratic bowls, L-BFGS-B and the Bernstein hull.
+* `jd_step`   -- JD witnesses, the point enclosure and the historical-witness
+                 step prediction. Pinned by exact rationals, known cubic roots
+                 and brute-force scans of det(A + sV).
 """

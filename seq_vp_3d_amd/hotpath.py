@@ -102,6 +102,7 @@ class PairState:
         self.u_t = z(*self.shape, 3)                 # trial
         self.theta_t = z(self.nfree)
         self.warped_t = z(*self.shape)
+        self.bgrad_t = z(*self.shape, 3)             # image gradient at the trial (promotion swaps it in)
         self.node_a = z(*self.shape, 3)
         self.node_b = z(*self.shape, 3)
         self.g = z(self.nfree)
@@ -252,7 +253,9 @@ class PairState:
                   _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
         self.cert.screen(self.u_t)
         self._certificate = None
-        _, _, bad = self.spline.warp(self.u_t, with_gradient=False, value=self.warped_t)
+        # value and gradient at once: the 64-tap gather dominates the warp, and an accepted trial then needs no
+        # second warp at promotion
+        _, _, bad = self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t)
         self._trial_bad = bad
         return self.trial_energies(screen)
 
@@ -301,11 +304,11 @@ class PairState:
         return TrialResult(True, "", energy, total, outer, gap, 0, band, cert)
 
     def promote(self, energy):
-        """Make the trial the accepted state (swap buffers) and sample the image gradient there."""
+        """Make the trial the accepted state (swap buffers; the trial's image gradient came with its warp)."""
         self.u, self.u_t = self.u_t, self.u
         self.theta, self.theta_t = self.theta_t, self.theta
         self.warped, self.warped_t = self.warped_t, self.warped
-        self.spline.warp(self.u, value=self.warped, gradient=self.bgrad)
+        self.bgrad, self.bgrad_t = self.bgrad_t, self.bgrad
         self.energy = energy
 
     # ---- outer-loop support ----------------------------------------------------------------------

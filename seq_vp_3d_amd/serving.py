@@ -28,15 +28,15 @@ from .solver import PGConfig, SequentialConfig, solve_sequential
 
 
 def footprint_bytes(shape_zyx):
-    """HBM bytes one resident request needs (upper estimate): 16 node fields of 3 doubles (state, trial, reference,
-    image gradient, node terms, snapshot, direction field), 8 node scalars (images, warped, spline coefficients,
+    """HBM bytes one resident request needs (upper estimate): 17 node fields of 3 doubles (state, trial, reference,
+    image gradients of the state and the trial, node terms, snapshot, direction field), 8 node scalars (images, warped, spline coefficients,
     certificate margins), 12 free vectors (theta, trial, gradient, direction, scale, diagonal, BB history,
     snapshot), the cell-local gradient buffer (48 doubles per cell) and 64 MiB of small buffers, plus 10%."""
     nz, ny, nx = shape_zyx
     n = nz * ny * nx
     c = (nz - 1) * (ny - 1) * (nx - 1)
     f = 3 * (nz - 2) * (ny - 2) * (nx - 2)
-    total = 8 * (16 * 3 * n + 8 * n + 12 * f + 48 * c) + (64 << 20)
+    total = 8 * (17 * 3 * n + 8 * n + 12 * f + 48 * c) + (64 << 20)
     return int(total * 1.1)
 
 

@@ -112,6 +112,7 @@ class PairState:
         self.diag = z(self.nfree)
         self.bad = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.dot_out = z(4)
+        self.scalars = z(12)        # slope | BB2 products (4) | step bound, read with one synchronisation
         self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
         config = certificate if certificate is not None else CertificateConfig(delta=problem.delta)
         if config.delta != problem.delta:
@@ -203,11 +204,13 @@ class PairState:
         nz, ny, nx = self.shape
         _lib.call("seqvp_metric_scale", _lib.ptr(self.diag), _lib.ptr(node), float(m), float(activation),
                   JD_AFFINE_FLOOR, nx, ny, nz, _lib.ptr(self.scale), _lib.ptr(self.bad), self._stream())
-        if int(self.bad.item()):
+        # one synchronisation for the validity flag and the extension record
+        flags = (self.bad.to(torch.float64) if stats is None else torch.cat((self.bad.to(torch.float64), stats))).tolist()
+        if int(flags[0]):
             raise FloatingPointError("GGN diagonal or node JD invalid")
         self.metric_info = dict(shift_alpha=alpha, a=a, b=bp, mass_diagonal=m)
         if stats is not None:
-            cand, lowered, lowered_any, mu_min = stats.tolist()
+            cand, lowered, lowered_any, mu_min = flags[1:5]
             self.metric_info["jd_affine"] = dict(activation=activation, extension=dict(
                 triggered="always", candidates=int(cand), lowered_cells=int(lowered),
                 lowered_cells_any=int(lowered_any), mu_min=mu_min))
@@ -370,6 +373,27 @@ class PairState:
                   _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
         sy, ss, yy2, yiy = out.tolist()
         return sy, float(np.sqrt(ss)), float(np.sqrt(yy2)), yiy
+
+    def direction_scalars(self, prev):
+        """(g.p, BB2 products for `prev` or None, domain step bound) with one host synchronisation: the three
+        reductions are enqueued back to back into separate slots of one small buffer."""
+        out = self.scalars
+        g, p = self.g, self.p
+        xs, ys = (ctypes.c_void_p * 1)(g.data_ptr()), (ctypes.c_void_p * 1)(p.data_ptr())
+        _lib.call("seqvp_dots", xs, ys, 1, g.numel(), _lib.ptr(out[0:1]), _lib.ptr(self.dot_ws), self.dot_ws.numel(),
+                  self._stream())
+        if prev is not None:
+            _lib.call("seqvp_bb2_products", _lib.ptr(self.theta), _lib.ptr(prev["theta"]), _lib.ptr(self.g),
+                      _lib.ptr(prev["g"]), _lib.ptr(prev["p"]), _lib.ptr(self.p), self.nfree, _lib.ptr(out[4:8]),
+                      _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        nz, ny, nx = self.shape
+        hx, hy, hz = self.grid.spacing
+        ox, oy, oz = self.grid.origin
+        _lib.call("seqvp_step_bound", _lib.ptr(self.p), _lib.ptr(self.u), ox, oy, oz, hx, hy, hz, nx, ny, nz,
+                  _lib.ptr(out[8:9]), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        v = out.tolist()
+        bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
+        return v[0], bb2, v[8]
 
     def step_bound(self):
         nz, ny, nx = self.shape

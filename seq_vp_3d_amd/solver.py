@@ -143,12 +143,13 @@ def _protect(config, value):
     return min(config.step_max, max(config.step_min, value))
 
 
-def step_proposal(st, config, previous):
-    """Accepted-only BB2 with the reference's fallbacks (poisson_gradient.py:955-990)."""
+def step_proposal(st, config, previous, products=None):
+    """Accepted-only BB2 with the reference's fallbacks (poisson_gradient.py:955-990). `products` are the
+    (s.y, |s|, |y|, y.iy) of `previous` when already read."""
     if previous is None:
         return _protect(config, config.initial_step), "initial"
     fallback = _protect(config, 2 * min(previous["tau"], config.step_max / 2))
-    sy, ns, ny, yiy = st.bb2_products(previous)
+    sy, ns, ny, yiy = st.bb2_products(previous) if products is None else products
     if not all(np.isfinite(v) for v in (sy, ns, ny, yiy)):
         return fallback, "fallback_nonfinite_secant"
     if sy <= 0:
@@ -214,13 +215,14 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 return finish("nonfinite_evaluation", str(exc))
             metric_ready = True
         st.direction(fp32=config.fp32_direction)
-        (slope,) = st.dots((st.g, st.p))
+        # the slope, the BB2 products and the domain bound come back together (the bound and the products are
+        # not used when the slope ends the solve)
+        slope, products, bound = st.direction_scalars(previous)
         if not np.isfinite(slope):
             return finish("nonfinite_evaluation", "PG directional derivative")
         if slope >= 0:
             return finish("non_descent_direction", "NON_DESCENT_DIRECTION")
-        proposal, rule = step_proposal(st, config, previous)
-        bound = st.step_bound()
+        proposal, rule = step_proposal(st, config, previous, products)
         if bound <= 0:
             return finish("domain_blocked", "NO_POSITIVE_DOMAIN_STEP")
         tau = min(proposal, .9 * bound)

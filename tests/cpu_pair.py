@@ -105,6 +105,10 @@ class CpuPairState:
             out.append(ojs.determinant_coefficients(A, V, self.m.delta))
         return np.concatenate(out)
 
+    def direction_scalars(self, prev):
+        (slope,) = self.dots((self.g, self.p))
+        return slope, (None if prev is None else self.bb2_products(prev)), self.step_bound()
+
     def step_bound(self):
         v = self.p.numpy().reshape((3,) + self.grid.interior_shape)
         pts = (self.grid.identity() + self.u)[1:-1, 1:-1, 1:-1]

@@ -1067,18 +1067,23 @@ __global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __rest
 // Node terms of the gradient and of the GGN diagonal (torch_ops.py:563-568, 784-791;
 // linearization.py:150-156, 362): image = qd (W - F) b; total = image + lam q (u - ref);
 // ggn = qd b^2 + lam q. Any output may be NULL.
+// One lane per vector component of a node row: a workgroup covers BLOCK of the 3 nx components of row (z, y), so
+// the vector fields are read and written with consecutive addresses and the row indices come from the block id
+// (no per-lane division); the node scalars are shared by the three components of a node (cache hits).
 __global__ void __launch_bounds__(BLOCK) node_terms_kernel(const double* __restrict__ warped,
                                                            const double* __restrict__ fixed,
                                                            const double* __restrict__ dw,
                                                            const double* __restrict__ bgrad,
                                                            const double* __restrict__ u,
                                                            const double* __restrict__ ref, double lam, double omega,
-                                                           Dims d, double* __restrict__ image,
+                                                           Dims d, int64_t blocks_per_row, double* __restrict__ image,
                                                            double* __restrict__ total, double* __restrict__ ggn) {
-  const int64_t nnode = d.nx * d.ny * d.nz;
-  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= nnode) return;
-  const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
+  const int64_t row = blockIdx.x / blocks_per_row;
+  const int64_t t = (blockIdx.x - row * blocks_per_row) * BLOCK + threadIdx.x;
+  if (t >= 3 * d.nx) return;
+  const int64_t y = row % d.ny, z = row / d.ny;
+  const int64_t x = t / 3;
+  const int64_t i = row * d.nx + x, e = 3 * row * d.nx + t;
   const double tx = (x == 0 || x == d.nx - 1) ? 0.5 : 1.0;
   const double ty = (y == 0 || y == d.ny - 1) ? 0.5 : 1.0;
   const double tz = (z == 0 || z == d.nz - 1) ? 0.5 : 1.0;
@@ -1086,14 +1091,11 @@ __global__ void __launch_bounds__(BLOCK) node_terms_kernel(const double* __restr
   const double qd = dw ? q * dw[i] : q;
   const double lq = lam * q;
   const double r = qd * (warped[i] - fixed[i]);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double b = bgrad[3 * i + c];
-    const double im = r * b;
-    if (image) image[3 * i + c] = im;
-    if (total) total[3 * i + c] = im + lq * (u[3 * i + c] - ref[3 * i + c]);
-    if (ggn) ggn[3 * i + c] = qd * (b * b) + lq;
-  }
+  const double b = bgrad[e];
+  const double im = r * b;
+  if (image) image[e] = im;
+  if (total) total[e] = im + lq * (u[e] - ref[e]);
+  if (ggn) ggn[e] = qd * (b * b) + lq;
 }
 
 }  // namespace seqvp
@@ -1330,8 +1332,10 @@ extern "C" int seqvp_node_terms(const double* warped, const double* fixed, const
   if (!warped || !fixed || !image_grad || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
   if (total && (!u || !ref)) return SEQVP_EINVAL;
   const Dims d = make_dims(nx, ny, nz);
-  hipLaunchKernelGGL(node_terms_kernel, dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, warped,
-                     fixed, data_weight, image_grad, u, ref, lam, hx * hy * hz, d, image, total, ggn);
+  const int64_t per_row = cdiv(3 * nx, BLOCK);
+  if (per_row * ny * nz > 0x7fffffff) return SEQVP_EINVAL;
+  hipLaunchKernelGGL(node_terms_kernel, dim3(per_row * ny * nz), dim3(BLOCK), 0, (hipStream_t)stream, warped, fixed,
+                     data_weight, image_grad, u, ref, lam, hx * hy * hz, d, per_row, image, total, ggn);
   return launch_status();
 }
 

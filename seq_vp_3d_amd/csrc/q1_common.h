@@ -89,10 +89,12 @@ __device__ __forceinline__ void make_edges(const T v[8][3], const T ih[3], Edges
 }
 
 // The bilinear blend of four parallel edges with explicit fused multiply-adds, so the derivative of a field is
-// the same bits whichever storage its edges come from (J(u) - J(ref) is exactly 0 when u == ref).
+// the same bits whichever storage its edges come from (J(u) - J(ref) is exactly 0 when u == ref). One multiply and
+// three chained fused multiply-adds: four VALU instructions, the fewest for four products (the sweeps issue
+// thousands of blends per cell and are VALU-issue bound).
 template <typename T>
 __device__ __forceinline__ T blend4(T e0, T e1, T e2, T e3, T w00, T w10, T w01, T w11) {
-  return fma(e0, w00, e1 * w10) + fma(e2, w01, e3 * w11);
+  return fma(e3, w11, fma(e2, w01, fma(e1, w10, e0 * w00)));
 }
 
 // V[r][j] = sum_k e[j][r][k] f(k0, s_o1) f(k1, s_o2) at reference point s = (sx, sy, sz).

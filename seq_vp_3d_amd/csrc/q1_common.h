@@ -111,6 +111,43 @@ __device__ __forceinline__ void point_grad(const EdgesT<T>& E, const T s[3], T V
   }
 }
 
+// ---- tabulated blend weights ------------------------------------------------------------------------------------
+// The blend weights (w00, w10, w01, w11) of direction j at point p of a 3x3x3 tensor rule depend only on p, which
+// is the same for every lane; tabulated in constant memory they are read with scalar loads instead of being
+// recomputed by six vector instructions per direction and point. Each entry is the expression point_grad
+// evaluates, rounded the same way, so the derivatives are the same bits.
+struct PointWeights {
+  double w[27][3][4];
+};
+
+constexpr PointWeights make_point_weights(double s0, double s1, double s2) {
+  PointWeights t{};
+  const double s[3] = {s0, s1, s2};
+  for (int p = 0; p < 27; ++p) {
+    const double c[3] = {s[p % 3], s[(p / 3) % 3], s[p / 9]};
+    for (int j = 0; j < 3; ++j) {
+      const double so1 = c[other1(j)], so2 = c[other2(j)];
+      t.w[p][j][0] = (1.0 - so1) * (1.0 - so2);
+      t.w[p][j][1] = so1 * (1.0 - so2);
+      t.w[p][j][2] = (1.0 - so1) * so2;
+      t.w[p][j][3] = so1 * so2;
+    }
+  }
+  return t;
+}
+
+__constant__ constexpr PointWeights GAUSS_W = make_point_weights(0x1.cda042f0236e0p-4, 0.5, 0x1.c64bf7a1fb924p-1);
+__constant__ constexpr PointWeights CLOSED_W = make_point_weights(0.0, 0.5, 1.0);
+
+// point_grad with the weights of one table row (w = TABLE.w[p]).
+__device__ __forceinline__ void point_grad_w(const Edges& E, const double (&w)[3][4], double V[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      V[r][j] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], w[j][0], w[j][1], w[j][2], w[j][3]);
+}
+
 typedef __attribute__((address_space(3))) double lds_double;
 
 // Edges of one field kept in LDS as a per-thread column (element (j, r, k) at col[((j*3 + r)*4 + k)*stride]):
@@ -124,6 +161,18 @@ __device__ __forceinline__ void stage_edges(const double v[8][3], const double i
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         col[((j * 3 + r) * 4 + k) * stride] = (v[edge_a1(j, k)][r] - v[edge_a0(j, k)][r]) * ih[j];
+}
+
+// point_grad_col with the weights of one table row.
+__device__ __forceinline__ void point_grad_col_w(const volatile lds_double* col, int stride, const double (&w)[3][4],
+                                                 double V[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const volatile lds_double* e = col + (j * 3 + r) * 4 * stride;
+      V[r][j] = blend4<double>(e[0], e[stride], e[2 * stride], e[3 * stride], w[j][0], w[j][1], w[j][2], w[j][3]);
+    }
 }
 
 // volatile: the column must be re-read at every point (a hoisted copy would bring the registers back); the LDS

@@ -136,10 +136,10 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
       gauss_point(p, sp);
       const double qw = k[K_QW + p];
       double V[3][3], W[3][3], C[3], Cr[3];
-      point_grad(Eu, sp, V);
+      point_grad_w(Eu, GAUSS_W.w[p], V);
       const double J = jacobian(V);
       curl_of(V, C);
-      point_grad_col(er, BLOCK, sp, W);
+      point_grad_col_w(er, BLOCK, GAUSS_W.w[p], W);
       const double Jr = jacobian(W);
       curl_of(W, Cr);
       const double dj = J - Jr;
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
         // difference as with sqrt(w) folded into the residuals; the point's four terms are summed first and
         // added to the double-double accumulator once.
         double Y[3][3], Ca[3];
-        point_grad(Eb, sp, Y);
+        point_grad_w(Eb, GAUSS_W.w[p], Y);
         const double Ja = jacobian(Y);
         curl_of(Y, Ca);
         const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
@@ -199,13 +199,13 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
       double sp[3];
       closed_point(p, sp);
       double V[3][3];
-      point_grad(Eu, sp, V);
+      point_grad_w(Eu, CLOSED_W.w[p], V);
       const double J = jacobian(V);
       m[0] = nan_min(m[0], J);
       val += barrier_value(J, w, delta, act, scale);
       if (WITH_BASE) {
         double Y[3][3];
-        point_grad(Eb, sp, Y);
+        point_grad_w(Eb, CLOSED_W.w[p], Y);
         const double Ja = jacobian(Y);
         m[1] = nan_min(m[1], Ja);
         if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
@@ -354,6 +354,19 @@ __device__ __forceinline__ void adj_zero(EdgeAdj& G) {
       for (int q = 0; q < 4; ++q) G.g[j][r][q] = 0.0;
 }
 
+// adj_accumulate with the weights of one table row.
+__device__ __forceinline__ void adj_accumulate_w(EdgeAdj& G, const double (&w)[3][4], const double X[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      G.g[j][r][0] += w[j][0] * X[r][j];
+      G.g[j][r][1] += w[j][1] * X[r][j];
+      G.g[j][r][2] += w[j][2] * X[r][j];
+      G.g[j][r][3] += w[j][3] * X[r][j];
+    }
+}
+
 __device__ __forceinline__ void adj_accumulate(EdgeAdj& G, const double s[3], const double X[3][3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -400,10 +413,10 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, 
     gauss_point(p, sp);
     const double qw = k[K_QW + p];
     double V[3][3], A[3][3], W[3][3], C[3], Cr[3];
-    point_grad(Eu, sp, V);
+    point_grad_w(Eu, GAUSS_W.w[p], V);
     const double J = cofactor_jacobian(V, A);
     curl_of(V, C);
-    point_grad_col(er, stride, sp, W);
+    point_grad_col_w(er, stride, GAUSS_W.w[p], W);
     const double Jr = jacobian(W);
     curl_of(W, Cr);
     const double jd = mu * qw * (J - Jr);
@@ -419,7 +432,7 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, 
     X[2][0] -= c1;
     X[1][0] += c2;
     X[0][1] -= c2;
-    adj_accumulate(G, sp, X);
+    adj_accumulate_w(G, GAUSS_W.w[p], X);
   }
 }
 
@@ -430,7 +443,7 @@ __device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __res
     double sp[3];
     closed_point(p, sp);
     double V[3][3];
-    point_grad(Eu, sp, V);
+    point_grad_w(Eu, CLOSED_W.w[p], V);
     // psi'(J) is exactly 0 at and above the activation: a point where every lane of the wave is
     // inactive contributes nothing and is skipped (wave-uniform branch).
     if (!__any(jacobian(V) < act)) continue;
@@ -442,7 +455,7 @@ __device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __res
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * b1;
-    adj_accumulate(G, sp, X);
+    adj_accumulate_w(G, CLOSED_W.w[p], X);
   }
 }
 
@@ -522,7 +535,7 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
     gauss_point(p, sp);
     const double qw = k[K_QW + p];
     double V[3][3], A[3][3];
-    point_grad(Eu, sp, V);
+    point_grad_w(Eu, GAUSS_W.w[p], V);
     cofactor_jacobian(V, A);
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
@@ -542,7 +555,7 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
     double sp[3];
     closed_point(p, sp);
     double V[3][3], A[3][3];
-    point_grad(Eu, sp, V);
+    point_grad_w(Eu, CLOSED_W.w[p], V);
     const double J = cofactor_jacobian(V, A);
     const double b2 = barrier_second(J, w, delta, act, scale);
     if (b2 != 0.0) {
@@ -622,7 +635,7 @@ __device__ __forceinline__ void cell_closed_jd(const double* __restrict__ u, con
   for (int p = 0; p < 27; ++p) {
     double sp[3], V[3][3];
     closed_point(p, sp);
-    point_grad(Eu, sp, V);
+    point_grad_w(Eu, CLOSED_W.w[p], V);
     J[p] = jacobian(V);
   }
 }
@@ -926,11 +939,11 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
       double sp[3];
       closed_point(p, sp);
       double F[3][3], V[3][3], AF[3][3], AV[3][3];
-      point_grad(Eu, sp, F);
+      point_grad_w(Eu, CLOSED_W.w[p], F);
       F[0][0] += 1.0;
       F[1][1] += 1.0;
       F[2][2] += 1.0;
-      point_grad(Ev, sp, V);
+      point_grad_w(Ev, CLOSED_W.w[p], V);
       cofactor_plain(F, AF);
       cofactor_plain(V, AV);
       const double c0 = ((F[0][0] * AF[0][0] + F[0][1] * AF[0][1]) + F[0][2] * AF[0][2]) - delta;

@@ -136,11 +136,36 @@ constexpr PointWeights make_point_weights(double s0, double s1, double s2) {
   return t;
 }
 
+// The same table in binary32 for the screening kernels: the nodes rounded to float, then the weights in float.
+struct PointWeightsF {
+  float w[27][3][4];
+};
+
+constexpr PointWeightsF make_point_weights_f(float s0, float s1, float s2) {
+  PointWeightsF t{};
+  const float s[3] = {s0, s1, s2};
+  for (int p = 0; p < 27; ++p) {
+    const float c[3] = {s[p % 3], s[(p / 3) % 3], s[p / 9]};
+    for (int j = 0; j < 3; ++j) {
+      const float so1 = c[other1(j)], so2 = c[other2(j)];
+      t.w[p][j][0] = (1.0f - so1) * (1.0f - so2);
+      t.w[p][j][1] = so1 * (1.0f - so2);
+      t.w[p][j][2] = (1.0f - so1) * so2;
+      t.w[p][j][3] = so1 * so2;
+    }
+  }
+  return t;
+}
+
 __constant__ constexpr PointWeights GAUSS_W = make_point_weights(0x1.cda042f0236e0p-4, 0.5, 0x1.c64bf7a1fb924p-1);
 __constant__ constexpr PointWeights CLOSED_W = make_point_weights(0.0, 0.5, 1.0);
+__constant__ constexpr PointWeightsF GAUSS_WF = make_point_weights_f((float)0x1.cda042f0236e0p-4, 0.5f,
+                                                                     (float)0x1.c64bf7a1fb924p-1);
+__constant__ constexpr PointWeightsF CLOSED_WF = make_point_weights_f(0.0f, 0.5f, 1.0f);
 
 // point_grad with the weights of one table row (w = TABLE.w[p]).
-__device__ __forceinline__ void point_grad_w(const Edges& E, const double (&w)[3][4], double V[3][3]) {
+template <typename T>
+__device__ __forceinline__ void point_grad_w(const EdgesT<T>& E, const T (&w)[3][4], T V[3][3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll

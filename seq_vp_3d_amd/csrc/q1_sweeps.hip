@@ -250,13 +250,12 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_f32_kernel(const double* __
     KahanF cj{0.f, 0.f}, cc{0.f, 0.f}, cd{0.f, 0.f};
 #pragma unroll 1
     for (int p = 0; p < 27; ++p) {
-      const float sp[3] = {(float)GAUSS_S[p % 3], (float)GAUSS_S[(p / 3) % 3], (float)GAUSS_S[p / 9]};
       const float qw = (float)k[K_QW + p];
       float V[3][3], W[3][3], C[3], Cr[3];
-      point_grad(Eu, sp, V);
+      point_grad_w(Eu, GAUSS_WF.w[p], V);
       const float J = jacobian(V);
       curl_of(V, C);
-      point_grad(Er, sp, W);
+      point_grad_w(Er, GAUSS_WF.w[p], W);
       const float Jr = jacobian(W);
       curl_of(W, Cr);
       const float dj = J - Jr;
@@ -265,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_f32_kernel(const double* __
       kahan_add(cc, (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2);
       if (WITH_BASE) {
         float Y[3][3], Ca[3];
-        point_grad(Eb, sp, Y);
+        point_grad_w(Eb, GAUSS_WF.w[p], Y);
         const float Ja = jacobian(Y);
         curl_of(Y, Ca);
         const float sm = sqrtf(mu * qw), sn = sqrtf(nu * qw);
@@ -312,15 +311,14 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_f32_kernel(const double*
     float m0 = __builtin_inff(), m1 = __builtin_inff();
 #pragma unroll 1
     for (int p = 0; p < 27; ++p) {
-      const float sp[3] = {(float)CLOSED_S[p % 3], (float)CLOSED_S[(p / 3) % 3], (float)CLOSED_S[p / 9]};
       float V[3][3];
-      point_grad(Eu, sp, V);
+      point_grad_w(Eu, CLOSED_WF.w[p], V);
       const float J = jacobian(V);
       m0 = (J != J || m0 != m0) ? __builtin_nanf("") : (J < m0 ? J : m0);
       kahan_add(val, barrier_value(J, w, delta, act, scale));
       if (WITH_BASE) {
         float Y[3][3];
-        point_grad(Eb, sp, Y);
+        point_grad_w(Eb, CLOSED_WF.w[p], Y);
         const float Ja = jacobian(Y);
         m1 = (Ja != Ja || m1 != m1) ? __builtin_nanf("") : (Ja < m1 ? Ja : m1);
         if (!(J >= act && Ja >= act)) kahan_add(chg, barrier_change(Ja, J, delta, act, scale));

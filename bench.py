@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--inner", type=int, default=5)
     ap.add_argument("--shape", type=str, default="160,192,224")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("driver", "fixed", "precision", "serve", "batch"), default="driver")
+    ap.add_argument("--workload", choices=("driver", "fixed", "precision", "serve", "batch", "file"),
+                    default="driver")
     ap.add_argument("--requests", type=int, default=24, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=2., help="serve workload: Poisson arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=4, help="serve/batch workloads: concurrent requests")
@@ -182,6 +183,54 @@ SERVE_SIZES = ((96, 96, 96), (128, 128, 128), (160, 192, 144), (160, 192, 224))
 BATCH_SIZES = ((96, 96, 96),) * 8 + ((128, 128, 128),) * 8 + ((160, 192, 144),) * 4 + ((160, 192, 224),) * 4
 
 
+def file_form(args, pair, dev):
+    """The pair-folder form: the synthetic pair is written as pair.npz + manifest.json (untimed), then run_pair --
+    hash check and load, upload, certified FP64 solve of `--steps` outer steps from u = 0, the final state's
+    certificate, final_native.npz/.json, summary.json, history.jsonl -- is timed end to end. The independent CPU
+    re-certification of the stored map is timed separately (it is an audit after the run)."""
+    import shutil
+    import tempfile
+    from pathlib import Path
+
+    import torch
+
+    from seq_vp_3d_amd import pairio
+
+    root = Path(tempfile.mkdtemp(prefix="seqvp_pair_"))
+    try:
+        folder = root / "pair"
+        t0 = time.perf_counter()
+        pairio.write_pair(folder, pair.fixed.cpu().numpy(), pair.moving.cpu().numpy(), pair="synthetic_native")
+        print(f"pair folder written in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        pairio.run_pair(folder, root / "warm", device=dev, route="native", max_outer=max(1, args.warmup))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        summary = pairio.run_pair(folder, root / "run", device=dev, route="native", max_outer=args.steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        cert = pairio.recertify(root / "run", folder, workers=8)
+        audit = time.perf_counter() - t1
+        outer = max(1, summary["outer_rows"])
+        line = dict(metric="ms/outer-step @%s" % "x".join(map(str, pair.grid.shape)), unit="ms/outer-step",
+                    value=round(wall * 1e3 / outer, 3), higher_is_better=False, n_gpus=1, steps=args.steps,
+                    warmup=args.warmup, dtype="f64", data="synthetic",
+                    config=dict(workload="pair folder -> run folder: load + certified FP64 solve from u = 0 + "
+                                         "final certificate + export, end to end", shape=list(pair.grid.shape)),
+                    status=summary["status"], outer_steps=summary["outer_rows"],
+                    native_commits=summary["native_commits"], wall_seconds=round(wall, 3),
+                    e2e_seconds=round(summary["e2e_seconds"], 3), input_seconds=round(summary["input_seconds"], 3),
+                    solve_seconds=round(summary["solve_call_wall_seconds"], 3),
+                    solve_ms_per_outer_step=round(summary["solve_call_wall_seconds"] * 1e3 / outer, 3),
+                    final_export_seconds=round(summary["final_export_seconds"], 3),
+                    stored_array_io_seconds=round(summary["stored_array_io_seconds"], 3),
+                    recertify_seconds=round(audit, 3), recertified=cert.status,
+                    recertified_min_jd_lower_bound=cert.min_jd_lower_bound, mse_ratio=summary["ratio"])
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+
+
 def serving_form(args, dev):
     """The batching (all requests at t = 0) and serving (Poisson arrivals) forms against serial one-pair-at-a-time
     execution of the same requests; each request runs `--outer` certified outer steps from u = 0 and its output is
@@ -281,6 +330,8 @@ def main():
     problem = Q1Problem(pair.grid, native_weights(shape), 0., .05)
     if args.workload == "precision":
         return precision_form(args, pair, problem, dev)
+    if args.workload == "file":
+        return file_form(args, pair, dev)
     st = PairState(problem, pair.fixed, pair.moving, dev)
     theta0 = torch.zeros(st.nfree, dtype=torch.float64, device=dev)
     if args.start == "half":

@@ -119,6 +119,7 @@ class PairState:
             raise ValueError("certificate delta must equal the problem's delta")
         self.cert = DeviceCertificate(g, self.device, config)
         self._certificate = None    # certificate of the current candidate (kept across its energy evaluations)
+        self.certificate = None     # certificate of the accepted state (None: not computed yet)
         self.poisson = None
         self._den = None
         self.energy = None          # Energies of the accepted state
@@ -148,8 +149,10 @@ class PairState:
         _lib.call("seqvp_restrict", _lib.ptr(field), nx, ny, nz, _lib.ptr(out), self._stream())
         return out
 
-    def set_state(self, theta=None, ref=None):
-        """Set the accepted map from theta (and optionally the reference) and evaluate it."""
+    def set_state(self, theta=None, ref=None, certificate=None):
+        """Set the accepted map from theta (and optionally the reference) and evaluate it. `certificate` is the
+        map's certificate when the caller has one (else state_certificate() computes it on demand)."""
+        self.certificate = certificate
         if ref is not None:
             self.ref.copy_(torch.as_tensor(ref, dtype=torch.float64))
         if theta is not None:
@@ -159,6 +162,13 @@ class PairState:
         if int(bad.item()):
             raise FloatingPointError("state outside the sampling support")
         self.energy = self.evaluate_energy(self.u, self.warped)
+
+    def state_certificate(self):
+        """The certificate of the accepted map: the one its trial or transfer passed, else screened now."""
+        if self.certificate is None:
+            self.cert.screen(self.u)
+            self.certificate = self.cert.finish()
+        return self.certificate
 
     def evaluate_energy(self, u, warped):
         q = self.q1
@@ -313,12 +323,13 @@ class PairState:
         self.warped, self.warped_t = self.warped_t, self.warped
         self.bgrad, self.bgrad_t = self.bgrad_t, self.bgrad
         self.energy = energy
+        self.certificate = self._certificate
 
     # ---- outer-loop support ----------------------------------------------------------------------
     def snapshot(self):
         """Device copies of the accepted state (the outer start)."""
         return dict(theta=self.theta.clone(), u=self.u.clone(), warped=self.warped.clone(), bgrad=self.bgrad.clone(),
-                    energy=self.energy)
+                    energy=self.energy, certificate=self.certificate)
 
     def restore(self, snap):
         self.theta.copy_(snap["theta"])
@@ -326,6 +337,7 @@ class PairState:
         self.warped.copy_(snap["warped"])
         self.bgrad.copy_(snap["bgrad"])
         self.energy = snap["energy"]
+        self.certificate = snap.get("certificate")
 
     def compare_to(self, snap):
         """(E(current) - E(snap), F(current) - F(snap), min gap): stable differences of the current state

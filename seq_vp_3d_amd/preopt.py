@@ -365,7 +365,7 @@ def certify_transfer(st, seed, alphas=TRANSFER_ALPHAS):
         attempts.append(dict(attempt_index=index, alpha=a, certificate_passed=ok, reason=reason,
                              min_jd_lower_bound=None if c is None else c.min_jd_lower_bound))
         if ok:
-            st.set_state(theta=st.theta_t.clone())
+            st.set_state(theta=st.theta_t.clone(), certificate=c)
             return a, attempts
         if reason not in ("JACOBIAN_VIOLATION", "subdivision_budget", "INVALID_DOMAIN"):
             break
@@ -373,10 +373,12 @@ def certify_transfer(st, seed, alphas=TRANSFER_ALPHAS):
 
 
 def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_outer=None, observer=None,
-                       sampler=None, report_errors=True):
+                       sampler=None, report_errors=True, budget=None, on_state=None):
     """The precision form: FP32 pyramid pre-optimisation, boundary taper, certified transfer, then the certified
     FP64 solve (solve_sequential) from the transferred state (the pre-optimizer route of
-    scripts/run_stage_3d_e19_mri.py:930-960 and multiresolution.prepare_transfer_problem).
+    scripts/run_stage_3d_e19_mri.py:930-960 and multiresolution.prepare_transfer_problem). The native solve keeps
+    the zero-displacement baseline (multiresolution.py:496-502); `budget()` is its wall-budget predicate and
+    `on_state(st)` is called once the PairState exists (before any outer row reaches `observer`).
     Returns (PairState, SequentialResult or None when no alpha certified, record)."""
     from .hotpath import PairState
     from .solver import PGConfig, SequentialConfig, solve_sequential
@@ -388,6 +390,8 @@ def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_o
     torch.cuda.synchronize(device)
     t1 = perf_counter()
     st = PairState(problem, fixed, moving, device, sampler=sampler)
+    if on_state is not None:
+        on_state(st)
     alpha, attempts = certify_transfer(st, seed)
     torch.cuda.synchronize(device)
     t2 = perf_counter()
@@ -396,6 +400,7 @@ def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_o
     if alpha is None:
         return st, None, record
     record["handoff_energy"] = st.energy.total
-    res = solve_sequential(st, SequentialConfig(), PGConfig(), observer=observer, max_outer=max_outer)
+    res = solve_sequential(st, SequentialConfig(), PGConfig(), observer=observer, max_outer=max_outer,
+                           budget=budget, baseline_at_zero=True)
     record["solve_seconds"] = perf_counter() - t2
     return st, res, record

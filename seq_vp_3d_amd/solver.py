@@ -125,18 +125,47 @@ class SequentialResult:
     history: list
     seconds: float
     energy: object = None
+    GD: float | None = None
+    GE: float | None = None
+    native_mse: float | None = None     # MSE of the baseline (u = 0 with baseline_at_zero, else the start)
+    final_mse: float | None = None
+    mse_ratio: float | None = None
+
+    @property
+    def success(self):
+        """Only the regularized gradient criterion is success (sequential.py:281)."""
+        return self.status == "converged_regularized_gradient"
+
+
+_WEIGHT_SUMS = {}
+
+
+def reported_mse(grid, image_energy):
+    """The reported MSE 2 D / sum q, the node-weight sum in long double (sequential.py:95-118), computed once per
+    grid shape and spacing (the sum over the full node array is ~0.1 s at the native size)."""
+    key = (tuple(grid.shape), tuple(float(h) for h in grid.spacing))
+    total = _WEIGHT_SUMS.get(key)
+    if total is None:
+        total = _WEIGHT_SUMS[key] = np.sum(grid.node_weights(), dtype=np.longdouble)
+    return float(2 * np.longdouble(image_energy) / total)
 
 
 CERTIFICATE_FIELDS = ("status", "reason", "min_jd_lower_bound", "n_fast_cells", "n_prescreen_cells",
                       "n_fallback_cells", "n_boxes", "n_uncertain", "n_infeasible")
 
 
+def _field(cert, key):
+    """One field of a FeasibilityResult or of a dict of its fields (None for no certificate)."""
+    if cert is None:
+        return None
+    return cert.get(key) if isinstance(cert, dict) else getattr(cert, key)
+
+
 def certificate_record(cert):
     """The certificate fields a history row keeps (a FeasibilityResult or a dict of its fields; None)."""
     if cert is None:
         return None
-    get = cert.get if isinstance(cert, dict) else (lambda k: getattr(cert, k))
-    return {k: get(k) for k in CERTIFICATE_FIELDS}
+    return {k: _field(cert, k) for k in CERTIFICATE_FIELDS}
 
 
 def _protect(config, value):
@@ -323,9 +352,11 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
 
 
 def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), budget=None, observer=None,
-                     max_outer=None):
+                     max_outer=None, baseline_at_zero=False):
     """Outer proximal loop (sequential.py:441-689): inner solve from the committed state, stable E and F
-    comparisons against the start, commit, self-reference refresh."""
+    comparisons against the start, commit, self-reference refresh. With baseline_at_zero (the native level of
+    the four-level route, multiresolution.py:496-502) G0, its threshold and the baseline MSE are those of u = 0
+    (sequential.py:396-419), whatever the start."""
     started = perf_counter()
     history = []
     memory = JDStepMemory()             # the level's witness memory, shared by its inner solves (sequential.py:214-215)
@@ -333,13 +364,23 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
     st.refresh_reference()
     GF, GD, GE = st.outer_norms()
     G0 = GF
+    native = reported_mse(st.grid, st.energy.image)
+    if baseline_at_zero and bool(torch.any(st.theta != 0)):
+        # F = D + B does not depend on the reference, so the zero map's GF needs only its image and barrier terms
+        start = st.snapshot()
+        st.set_state(theta=torch.zeros_like(st.theta))
+        G0 = st.outer_norms()[0]
+        native = reported_mse(st.grid, st.energy.image)
+        st.restore(start)
     threshold = 1e-8 + 1e-6 * G0
     accepted_count, small_steps = 0, 0
     limit = config.max_iterations if max_outer is None else min(config.max_iterations, max_outer)
 
     def finish(status, reason):
+        final = reported_mse(st.grid, st.energy.image)
         return SequentialResult(status, reason, accepted_count, G0, GF, threshold, history,
-                                perf_counter() - started, st.energy)
+                                perf_counter() - started, st.energy, GD, GE, native, final,
+                                None if native == 0 else final / native)
 
     while True:
         if budget is not None and budget():
@@ -373,21 +414,46 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
             accepted = bool(eligible and cert and not same)
             reason = ("ACCEPTED" if accepted else "INNER_FAILED_NOT_COMMITTED" if not eligible
                       else "NO_PROGRESS" if same else "DESCENT_CERTIFICATE_FAILED")
-        row = dict(outer_i=len(history), E_start=start["energy"].total, F_start=start["energy"].outer,
-                   E_candidate=st.energy.total if inner.accepted_steps else None,
+        valid = inner.accepted_steps > 0
+        se, ce = start["energy"], st.energy
+        cc = st.certificate if valid else None
+        candidate_mse = reported_mse(st.grid, ce.image) if valid else None
+        row = dict(outer_i=len(history), E_start=se.total, E_candidate=ce.total if valid else None,
+                   F_start=se.outer, F_candidate=ce.outer if valid else None,
+                   D_start=se.image, D_candidate=ce.image if valid else None,
+                   B_start=se.barrier, B_candidate=ce.barrier if valid else None,
+                   penalty_proximity=ce.proximity if valid else None, penalty_jd=ce.jd if valid else None,
+                   penalty_curl=ce.curl if valid else None,
                    actual_reduction_E=None if total is None else total[0],
                    resolution_E=None if total is None else total[1],
                    actual_reduction_F=None if fixed is None else fixed[0],
                    resolution_F=None if fixed is None else fixed[1],
+                   GF_initial=G0, GF_threshold=threshold,
                    GF_before=GF, GF_candidate=GFa, GD_candidate=GDa, GE_candidate=GEa, forcing_eta=float(eta),
-                   forcing_target=target, inner_status=inner.status, inner_reason=inner.reason,
+                   forcing_target=target, forcing_passed=bool(valid and GEa is not None and GEa <= target),
+                   inner_status=inner.status, inner_reason=inner.reason,
                    inner_accepted_steps=inner.accepted_steps, inner_trials=inner.trials, outer_accepted=accepted,
-                   reason=reason, step_ratio=relative, seconds=perf_counter() - tick)
+                   reason=reason, step_ratio=relative,
+                   candidate_min_jd_lower_bound=_field(cc, "min_jd_lower_bound"),
+                   candidate_sampled_min_jd=_field(cc, "sampled_min_jd"),
+                   candidate_mse_ratio=None if candidate_mse is None or native == 0 else candidate_mse / native,
+                   seconds=perf_counter() - tick, elapsed_seconds=perf_counter() - started)
+        time_limited = False
+        if accepted and budget is not None and budget():
+            # no commit once the budget has expired, checked right before the state transition (sequential.py:627-637)
+            accepted, time_limited, reason = False, True, "WALL_BUDGET_NOT_COMMITTED"
+            row.update(outer_accepted=False, reason=reason)
+        committed = ce if accepted else se
+        committed_mse = candidate_mse if accepted else reported_mse(st.grid, se.image)
+        row.update(committed_mse_ratio=None if native == 0 else committed_mse / native,
+                   committed_GF=GFa if accepted else GF, committed_E=committed.total, committed_F=committed.outer)
         history.append(row)
         if observer is not None:
             observer(row)
         if not accepted:
             st.restore(start)
+            if time_limited:
+                return finish("outer_time_budget", reason)
             if inner.status == "precision_stagnated":
                 return finish("outer_precision_stagnated", reason)
             if inner.status in ("domain_blocked", "feasibility_uncertain"):
@@ -398,6 +464,7 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
         accepted_count += 1
         GF, GD, GE = GFa, GDa, GEa
         small_steps = small_steps + 1 if relative <= config.step_tol else 0
-        if GF <= threshold or accepted_count >= limit or small_steps >= config.stall_steps:
+        if (GF <= threshold or accepted_count >= limit or small_steps >= config.stall_steps
+                or (budget is not None and budget())):
             continue
         st.refresh_reference()

@@ -42,6 +42,7 @@ class CpuPairState:
         return u, value, grad
 
     def set_state(self, theta):
+        self.certificate = None
         self.theta = self._t(theta)
         self.u, self.warped, self.bgrad = self._eval(self.theta.numpy())
         self.energy = self.evaluate_energy(self.u, self.warped)
@@ -158,7 +159,7 @@ class CpuPairState:
         if folded:
             return TrialResult(False, "JACOBIAN_VIOLATION", None, certificate=cert)
         value, grad = osp.sample(self.spline, pts)
-        self._trial = (u, value, grad)
+        self._trial = (u, value, grad, cert)
         e = self.evaluate_energy(u, value)
         ref = self.ref.numpy()
         _, _, sdiff = self.m.soft_energy(u, ref, self.u)
@@ -175,18 +176,19 @@ class CpuPairState:
 
     def promote(self, energy):
         self.theta = self.theta_t.clone()
-        self.u, self.warped, self.bgrad = self._trial
+        self.u, self.warped, self.bgrad, self.certificate = self._trial
         self.energy = energy
 
     # outer -------------------------------------------------------------------------------------
     def snapshot(self):
         return dict(theta=self.theta.clone(), u=self.u.copy(), warped=self.warped.copy(), bgrad=self.bgrad.copy(),
-                    energy=self.energy)
+                    energy=self.energy, certificate=self.certificate)
 
     def restore(self, snap):
         self.theta = snap["theta"].clone()
         self.u, self.warped, self.bgrad = snap["u"].copy(), snap["warped"].copy(), snap["bgrad"].copy()
         self.energy = snap["energy"]
+        self.certificate = snap["certificate"]
 
     def compare_to(self, snap):
         e0, e1 = snap["energy"], self.energy

@@ -72,3 +72,27 @@ def test_bb2_fallbacks():
     assert step_proposal(S((1., 1., 1., -2.)), cfg, prev) == (.6, "fallback_nonpositive_metric_curvature")
     assert step_proposal(S((1., 1., 1., 1e-12)), cfg, prev) == (10., "bb2")
     assert step_proposal(S((1., 1., 1., 1.)), cfg, None) == (1., "initial")
+
+
+def test_zero_baseline_and_reported_mse(pair):
+    """baseline_at_zero: G0 and the baseline MSE are the zero map's whatever the start (sequential.py:396-419);
+    the MSE is 2 D / sum q, here checked against sum q (W - F)^2 / sum q of the zero map, where W is the spline
+    of the moving image at the nodes (= the moving image)."""
+    from seq_vp_3d_amd.problem import make_grid_3d
+    spec, p = pair
+    zero = CpuPairState(spec.shape, (1., 1., 1.), native_weights(spec.shape), p.fixed.numpy(), p.moving.numpy())
+    zero.refresh_reference()
+    g0_zero = zero.outer_norms()[0]
+    q = make_grid_3d(nx=spec.shape[2], ny=spec.shape[1], nz=spec.shape[0], hx=1., hy=1., hz=1.).node_weights()
+    mse_zero = float(np.sum(q * (p.moving.numpy() - p.fixed.numpy()) ** 2) / np.sum(q))
+    st = _state(spec, p)
+    res_start = solve_sequential(st, SequentialConfig(), PGConfig(), max_outer=0)
+    st = _state(spec, p)
+    res_zero = solve_sequential(st, SequentialConfig(), PGConfig(), max_outer=0, baseline_at_zero=True)
+    assert res_zero.G0 == g0_zero and res_start.G0 != g0_zero
+    assert res_zero.threshold == 1e-8 + 1e-6 * g0_zero
+    assert abs(res_zero.native_mse - mse_zero) <= 1e-9 * mse_zero
+    assert res_start.native_mse == res_start.final_mse != res_zero.native_mse      # the start is not u = 0
+    assert res_zero.mse_ratio == res_zero.final_mse / res_zero.native_mse
+    # the state is back where it started
+    np.testing.assert_array_equal(st.theta.numpy(), _state(spec, p).theta.numpy())

@@ -35,13 +35,13 @@ FP64_PEAK_TFLOPS = 78.6
 # (--copy-probe: fetch x2.0, write x1.0)
 TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_soft_energy": 986e6,
-    "seqvp_q1_barrier_energy": 365e6,
-    "seqvp_q1_gradient": 3667e6,
-    "seqvp_q1_joint": 6359e6,
-    "seqvp_jd_root": 369e6,
-    "seqvp_certificate_screen": 282e6,
-    "seqvp_spline_warp": 532e6,
-    "seqvp_dst_poisson": 5928e6,
+    "seqvp_q1_barrier_energy": 362e6,
+    "seqvp_q1_gradient": 3666e6,
+    "seqvp_q1_joint": 6353e6,
+    "seqvp_jd_root": 361e6,
+    "seqvp_certificate_screen": 283e6,
+    "seqvp_spline_warp": 605e6,
+    "seqvp_dst_poisson": 2540e6,
 }
 
 

@@ -29,6 +29,17 @@ FLOPS_PER_UNIT = {
     "seqvp_jd_root": 4015.,
     "seqvp_spline_warp": 1078.,
 }
+# FP64 operations the build's kernels actually execute per cell (per node for the sampler): SQ_INSTS_VALU_FLOPS_FP64
+# x 64 lanes per dispatch / units (rocprofv3 --pmc, profiles/valu_issue_native_pmc.md). The edge-difference
+# geometry and the tabulated blend weights do the reference's work with about half of its operation count.
+EXECUTED_FLOPS_PER_UNIT = {
+    "seqvp_q1_soft_energy": 8411.,
+    "seqvp_q1_barrier_energy": 4603.,
+    "seqvp_q1_gradient": 10197. + 24.,
+    "seqvp_q1_ggn_diagonal": 14167.,
+    "seqvp_jd_root": 6536.,
+    "seqvp_spline_warp": 967.,
+}
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch at 160x192x224 from the FETCH_SIZE and WRITE_SIZE counter runs (rocprofv3 --pmc, one counter
 # per run, profiles/hbm_traffic_native_pmc.md), each counter scaled by its reading for a copy of known size
@@ -378,6 +389,7 @@ def main():
     units = nnode if dominant == "seqvp_spline_warp" else ncell
     calls, mean_ms = kern[dominant]
     achieved = FLOPS_PER_UNIT[dominant] * units / (mean_ms * 1e-3) / 1e12
+    executed = EXECUTED_FLOPS_PER_UNIT[dominant] * units / (mean_ms * 1e-3) / 1e12
     line = dict(
         metric="ms/outer-step @160x192x224", value=round(ms_per_step, 3), unit="ms/outer-step", n_gpus=1,
         steps=args.steps, warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=False,
@@ -393,9 +405,11 @@ def main():
         roofline=dict(bound="mfma", achieved=round(achieved, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                       frac=round(achieved / FP64_PEAK_TFLOPS, 4),
                       traffic=TRAFFIC_BYTES_NATIVE.get(dominant) if shape == (160, 192, 224) else None,
-                      kernel=dominant,
-                      mean_ms=round(mean_ms, 4), note="FP64 vector peak (equal to the FP64 matrix peak on MI355X); "
-                                                        "achieved = algorithmic flops per launch / mean launch time"),
+                      kernel=dominant, mean_ms=round(mean_ms, 4),
+                      executed=round(executed, 2), executed_frac=round(executed / FP64_PEAK_TFLOPS, 4),
+                      note="FP64 vector peak (equal to the FP64 matrix peak on MI355X); achieved = algorithmic flops "
+                           "(the reference's operation count) per launch / mean launch time; executed = the FP64 "
+                           "operations this kernel performs (PMC) / the same time"),
         kernels={k: dict(calls=c, mean_ms=round(ms, 4)) for k, (c, ms) in sorted(kern.items())},
     )
     if not args.no_cpu_baseline:

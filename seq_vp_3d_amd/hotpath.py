@@ -112,7 +112,10 @@ class PairState:
         self.diag = z(self.nfree)
         self.bad = torch.zeros(1, dtype=torch.int64, device=self.device)
         self.dot_out = z(4)
-        self.scalars = z(12)        # slope | BB2 products (4) | step bound, read with one synchronisation
+        # slope | gradient norm^2 | - | - | BB2 products (4) | step bound | step ratio (2) | -, read with one
+        # synchronisation
+        self.scalars = z(12)
+        self._prev = None           # (theta, g, p) of the last accepted state, kept for the next BB2 products
         self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
         config = certificate if certificate is not None else CertificateConfig(delta=problem.delta)
         if config.delta != problem.delta:
@@ -316,14 +319,27 @@ class PairState:
             band = FP32_EPS * (FP32_BAND_SOFT * soft_sum + FP32_BAND_BARRIER * bar_sum)
         return TrialResult(True, "", energy, total, outer, gap, 0, band, cert)
 
-    def promote(self, energy):
-        """Make the trial the accepted state (swap buffers; the trial's image gradient came with its warp)."""
+    def promote(self, energy, keep_history=False):
+        """Make the trial the accepted state (swap buffers; the trial's image gradient came with its warp). With
+        `keep_history` the replaced state's (theta, g, p) are returned for the next BB2 products, kept by moving
+        buffers rather than copying them: g and p change places with history slots (the next gradient and
+        direction write the freed ones), and the replaced theta leaves theta_t for a slot before the next trial."""
+        if keep_history:
+            if self._prev is None:
+                self._prev = dict(theta=torch.empty_like(self.theta), g=torch.empty_like(self.g),
+                                  p=torch.empty_like(self.p))
+            self.g, self._prev["g"] = self._prev["g"], self.g
+            self.p, self._prev["p"] = self._prev["p"], self.p
         self.u, self.u_t = self.u_t, self.u
         self.theta, self.theta_t = self.theta_t, self.theta
         self.warped, self.warped_t = self.warped_t, self.warped
         self.bgrad, self.bgrad_t = self.bgrad_t, self.bgrad
         self.energy = energy
         self.certificate = self._certificate
+        if not keep_history:
+            return None
+        self.theta_t, self._prev["theta"] = self._prev["theta"], self.theta_t
+        return dict(self._prev)
 
     # ---- outer-loop support ----------------------------------------------------------------------
     def snapshot(self):
@@ -386,14 +402,18 @@ class PairState:
         sy, ss, yy2, yiy = out.tolist()
         return sy, float(np.sqrt(ss)), float(np.sqrt(yy2)), yiy
 
-    def direction_scalars(self, prev):
-        """(g.p, BB2 products for `prev` or None, domain step bound) with one host synchronisation: the three
-        reductions are enqueued back to back into separate slots of one small buffer."""
+    def direction_scalars(self, prev, norm=False, ratio=False):
+        """(g.p, BB2 products for `prev` or None, domain step bound, gradient norm or None, ratio of the last
+        accepted step or None) with one host synchronisation: the reductions are enqueued back to back into
+        separate slots of one small buffer. `ratio` reads the slot step_ratio_launch filled."""
         out = self.scalars
         g, p = self.g, self.p
         xs, ys = (ctypes.c_void_p * 1)(g.data_ptr()), (ctypes.c_void_p * 1)(p.data_ptr())
         _lib.call("seqvp_dots", xs, ys, 1, g.numel(), _lib.ptr(out[0:1]), _lib.ptr(self.dot_ws), self.dot_ws.numel(),
                   self._stream())
+        if norm:
+            _lib.call("seqvp_dots", xs, xs, 1, g.numel(), _lib.ptr(out[1:2]), _lib.ptr(self.dot_ws),
+                      self.dot_ws.numel(), self._stream())
         if prev is not None:
             _lib.call("seqvp_bb2_products", _lib.ptr(self.theta), _lib.ptr(prev["theta"]), _lib.ptr(self.g),
                       _lib.ptr(prev["g"]), _lib.ptr(prev["p"]), _lib.ptr(self.p), self.nfree, _lib.ptr(out[4:8]),
@@ -405,7 +425,8 @@ class PairState:
                   _lib.ptr(out[8:9]), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
         v = out.tolist()
         bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
-        return v[0], bb2, v[8]
+        G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None
+        return v[0], bb2, v[8], G, (self._ratio(v[9], v[10]) if ratio else None)
 
     def step_bound(self):
         nz, ny, nx = self.shape
@@ -420,6 +441,18 @@ class PairState:
         _lib.call("seqvp_max_abs_diff", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.dot_out),
                   _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
         top, bmax = self.dot_out[:2].tolist()
+        return self._ratio(top, bmax)
+
+    def _ratio(self, top, bmax):
         length = float(np.linalg.norm([ax[-1] - ax[0] for ax in self.grid.axes]))
         return float(np.float64(top) / max(length, bmax))
+
+    def step_ratio_launch(self, a, b):
+        """step_ratio(a, b) left on the device, read by the next direction_scalars(ratio=True) or read_step_ratio."""
+        _lib.call("seqvp_max_abs_diff", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.scalars[9:11]),
+                  _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+
+    def read_step_ratio(self):
+        top, bmax = self.scalars[9:11].tolist()
+        return self._ratio(top, bmax)
 

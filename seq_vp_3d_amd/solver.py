@@ -209,7 +209,9 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
     def finish(status, reason):
         return InnerResult(status, reason, accepted_steps, G, trials, history)
 
+    ratio_pending = False       # the last accepted step's ratio is on the device, read with the next scalars
     while True:
+        scalars = None
         try:
             if accepted_steps >= config.max_iterations:
                 # this gradient ends the solve (every exit below returns): take it from the joint outer
@@ -219,9 +221,21 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 G = st.outer_cache[2]
             else:
                 st.gradient()
-                G = st.gradient_norm()
+                if metric_ready:
+                    # the direction and its scalars are enqueued before the gradient checks and read with the
+                    # gradient norm (and the last step's ratio) in one synchronisation; when a check below ends
+                    # the solve they are simply not used
+                    st.direction(fp32=config.fp32_direction)
+                    scalars = st.direction_scalars(previous, norm=True, ratio=ratio_pending)
+                    G = scalars[3]
+                else:
+                    G = st.gradient_norm()
         except FloatingPointError as exc:
             return finish("nonfinite_evaluation", str(exc))
+        if ratio_pending:
+            relative = scalars[4] if scalars is not None else st.read_step_ratio()
+            small_steps = small_steps + 1 if relative <= config.step_tol else 0
+            ratio_pending = False
         if not np.isfinite(G):
             return finish("nonfinite_evaluation", "nonfinite gradient")
         if G <= target:
@@ -243,10 +257,12 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             except FloatingPointError as exc:
                 return finish("nonfinite_evaluation", str(exc))
             metric_ready = True
-        st.direction(fp32=config.fp32_direction)
-        # the slope, the BB2 products and the domain bound come back together (the bound and the products are
-        # not used when the slope ends the solve)
-        slope, products, bound = st.direction_scalars(previous)
+        if scalars is None:
+            st.direction(fp32=config.fp32_direction)
+            # the slope, the BB2 products and the domain bound come back together (the bound and the products
+            # are not used when the slope ends the solve)
+            scalars = st.direction_scalars(previous)
+        slope, products, bound = scalars[:3]
         if not np.isfinite(slope):
             return finish("nonfinite_evaluation", "PG directional derivative")
         if slope >= 0:
@@ -327,11 +343,11 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             if reason.startswith("NONFINITE"):
                 return finish("nonfinite_evaluation", reason)
             if accepted:
-                old = dict(theta=st.theta.clone(), g=st.g.clone(), p=st.p.clone(), tau=tau)
-                relative = st.step_ratio(st.theta_t, st.theta)
-                small_steps = small_steps + 1 if relative <= config.step_tol else 0
-                st.promote(tr.energy)
-                previous = old
+                # the step ratio (small-step count) is read at the next synchronisation, before any check uses it
+                st.step_ratio_launch(st.theta_t, st.theta)
+                ratio_pending = True
+                previous = st.promote(tr.energy, keep_history=True)
+                previous["tau"] = tau
                 accepted_steps += 1
                 break
             global_attempts += 1

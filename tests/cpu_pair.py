@@ -106,9 +106,16 @@ class CpuPairState:
             out.append(ojs.determinant_coefficients(A, V, self.m.delta))
         return np.concatenate(out)
 
-    def direction_scalars(self, prev):
+    def direction_scalars(self, prev, norm=False, ratio=False):
         (slope,) = self.dots((self.g, self.p))
-        return slope, (None if prev is None else self.bb2_products(prev)), self.step_bound()
+        return (slope, (None if prev is None else self.bb2_products(prev)), self.step_bound(),
+                self.gradient_norm() if norm else None, self._ratio if ratio else None)
+
+    def step_ratio_launch(self, a, b):
+        self._ratio = self.step_ratio(a, b)
+
+    def read_step_ratio(self):
+        return self._ratio
 
     def step_bound(self):
         v = self.p.numpy().reshape((3,) + self.grid.interior_shape)
@@ -174,10 +181,12 @@ class CpuPairState:
         outer = img + bar[2]
         return TrialResult(True, "", e, total, outer, min(bar[1], bar[3]) - self.m.delta, 0, None, cert)
 
-    def promote(self, energy):
+    def promote(self, energy, keep_history=False):
+        prev = dict(theta=self.theta.clone(), g=self.g.clone(), p=self.p.clone()) if keep_history else None
         self.theta = self.theta_t.clone()
         self.u, self.warped, self.bgrad, self.certificate = self._trial
         self.energy = energy
+        return prev
 
     # outer -------------------------------------------------------------------------------------
     def snapshot(self):

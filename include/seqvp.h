@@ -354,6 +354,13 @@ int seqvp_certificate_finish(const double* screen, int64_t ncell, const double* 
                              int64_t n, const double* spacing, double delta, int64_t tile, int max_depth,
                              int64_t max_boxes, int workers, SeqvpCertificate* out);
 
+/* HOST. Edge-hull margins and flags of every cell of a host map u (nz, ny, nx, 3) on `workers` CPU threads, the
+ * first stage of seqvp_certify_map_host (replaces parallel_margins, compiled_enclosure.py:193-227, 336-345):
+ * margin[c] = lo(J_hull - delta), flag[c] = 1 when an interval step was nonfinite or inverted. Four cells of a row
+ * per AVX2 evaluation when the CPU has AVX2; every unflagged margin is bitwise the scalar one. */
+int seqvp_edge_hull_host(const double* u, int64_t nx, int64_t ny, int64_t nz, const double* spacing, double delta,
+                         int workers, double* margin, uint8_t* flag);
+
 /* HOST. The whole certificate of a host map u (nz, ny, nx, 3) on the CPU (certify_map, feasibility.py:395-431 and
  * _certify_map_parallel): nonfinite/boundary/domain checks against the axes ax/ay/az, edge hull, prescreen (D =
  * host derivative table, NULL = none), Bernstein fallback. The independent re-certification of an output. */

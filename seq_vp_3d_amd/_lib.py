@@ -64,6 +64,7 @@ SIGNATURES = {
     "seqvp_bernstein_arrays": [c_vp, c_i64, c_vp, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp],
     "seqvp_bernstein_cell": [c_vp, c_vp, c_dbl, c_int, c_i64, c_vp],
     "seqvp_certificate_finish": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_i64, c_int, c_i64, c_int, c_vp],
+    "seqvp_edge_hull_host": [c_vp, c_i64, c_i64, c_i64, c_vp, c_dbl, c_int, c_vp, c_vp],
     "seqvp_certify_map_host": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_i64, c_int, c_i64,
                                c_int, c_vp],
     "seqvp_preopt_workspace_bytes": [c_i64, c_i64, c_i64, c_int],

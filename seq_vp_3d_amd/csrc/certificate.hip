@@ -20,6 +20,7 @@
 
 #include "q1_common.h"
 #include "interval.h"
+#include "interval_simd.h"
 #include "prescreen_cell.h"
 #include "abi_util.h"
 
@@ -687,6 +688,68 @@ static void early(SeqvpCertificate* out, int status, int reason, int64_t ncell) 
   out->n_uncertain = ncell;
 }
 
+// Edge-hull margin and flag of cell c of u (x fastest) with the scalar operations.
+static inline void hull_cell(const double* u, const Dims& d, int64_t nx, int64_t ny, int64_t c, const double h[3],
+                             double delta, double* margin, uint8_t* flag) {
+  const int64_t x0 = c % d.cx, y0 = (c / d.cx) % d.cy, z0 = c / (d.cx * d.cy);
+  const int64_t b = (z0 * ny + y0) * nx + x0;
+  double uv[8][3];
+  for (int a = 0; a < 8; ++a) {
+    const double* p = u + 3 * (b + (a & 1) + ((a >> 1) & 1) * nx + ((a >> 2) & 1) * nx * ny);
+    uv[a][0] = p[0];
+    uv[a][1] = p[1];
+    uv[a][2] = p[2];
+  }
+  bool ok = true;
+  margin[c] = iv::edge_hull_margin<true>(uv, h, delta, ok);   // unflagged margins: bitwise
+  flag[c] = ok ? 0 : 1;
+}
+
+// Cells [c0, c1): four consecutive cells of a row per AVX2 evaluation (the same bits as hull_cell), the row ends
+// one by one.
+__attribute__((target("avx2"))) static void hull_range_avx2(const double* u, const Dims& d, int64_t nx, int64_t ny,
+                                                            int64_t c0, int64_t c1, const double h[3],
+                                                            double delta, double* margin, uint8_t* flag) {
+  for (int64_t c = c0; c < c1;) {
+    const int64_t x0 = c % d.cx;
+    if (x0 + 4 > d.cx || c + 4 > c1) {
+      hull_cell(u, d, nx, ny, c, h, delta, margin, flag);
+      ++c;
+      continue;
+    }
+    const int64_t y0 = (c / d.cx) % d.cy, z0 = c / (d.cx * d.cy);
+    const int64_t b = (z0 * ny + y0) * nx + x0;
+    iv4::d4 uv[8][3];
+    for (int a = 0; a < 8; ++a) {
+      const double* p = u + 3 * (b + (a & 1) + ((a >> 1) & 1) * nx + ((a >> 2) & 1) * nx * ny);
+      for (int r = 0; r < 3; ++r) uv[a][r] = iv4::d4{p[r], p[3 + r], p[6 + r], p[9 + r]};
+    }
+    iv4::l4 ok = (iv4::l4)-1;
+    const iv4::d4 m = iv4::edge_hull_margin(uv, h, delta, ok);
+    for (int l = 0; l < 4; ++l) {
+      margin[c + l] = m[l];
+      flag[c + l] = ok[l] ? 0 : 1;
+    }
+    c += 4;
+  }
+}
+
+// Edge-hull margins and flags of every cell, in chunks over `workers` threads.
+static void hull_all(const double* u, const Dims& d, int64_t nx, int64_t ny, const double h[3], double delta,
+                     int workers, double* margin, uint8_t* flag) {
+  const int64_t chunk = 16384;
+  const int64_t nchunks = (d.ncell + chunk - 1) / chunk;
+  const bool simd = iv4::avx2_available();
+  parallel_for(nchunks, workers, [&](int64_t k) {
+    const int64_t c0 = k * chunk, c1 = std::min(d.ncell, (k + 1) * chunk);
+    if (simd) {
+      hull_range_avx2(u, d, nx, ny, c0, c1, h, delta, margin, flag);
+    } else {
+      for (int64_t c = c0; c < c1; ++c) hull_cell(u, d, nx, ny, c, h, delta, margin, flag);
+    }
+  });
+}
+
 }  // namespace host
 }  // namespace seqvp
 
@@ -836,6 +899,13 @@ extern "C" int seqvp_certificate_finish(const double* screen, int64_t ncell, con
   return SEQVP_OK;
 }
 
+extern "C" int seqvp_edge_hull_host(const double* u, int64_t nx, int64_t ny, int64_t nz, const double* spacing,
+                                    double delta, int workers, double* margin, uint8_t* flag) {
+  if (!u || !spacing || !margin || !flag || nx < 2 || ny < 2 || nz < 2) return SEQVP_EINVAL;
+  host::hull_all(u, make_dims(nx, ny, nz), nx, ny, spacing, delta, workers, margin, flag);
+  return SEQVP_OK;
+}
+
 extern "C" int seqvp_certify_map_host(const double* u, int64_t nx, int64_t ny, int64_t nz, const double* ax,
                                       const double* ay, const double* az, const double* spacing, double delta,
                                       const double* D, int64_t tile, int max_depth, int64_t max_boxes, int workers,
@@ -844,82 +914,88 @@ extern "C" int seqvp_certify_map_host(const double* u, int64_t nx, int64_t ny, i
       max_boxes < 1)
     return SEQVP_EINVAL;
   const Dims d = make_dims(nx, ny, nz);
-  const int64_t nn = nx * ny * nz;
-  // NONFINITE_STATE (status uncertain), NONZERO_BOUNDARY, sampling domain (feasibility.py:413-431)
-  for (int64_t i = 0; i < 3 * nn; ++i)
-    if (!iv::is_finite(u[i])) {
-      host::early(out, host::UNCERTAIN, SEQVP_CERT_NONFINITE_STATE, d.ncell);
-      return SEQVP_OK;
-    }
-  for (int64_t z = 0; z < nz; ++z)
-    for (int64_t y = 0; y < ny; ++y)
-      for (int64_t x = 0; x < nx; ++x) {
-        const bool face = z == 0 || z == nz - 1 || y == 0 || y == ny - 1 || x == 0 || x == nx - 1;
-        if (!face) continue;
-        const double* p = u + 3 * ((z * ny + y) * nx + x);
-        if (p[0] != 0.0 || p[1] != 0.0 || p[2] != 0.0) {
-          host::early(out, host::INFEASIBLE, SEQVP_CERT_NONZERO_BOUNDARY, d.ncell);
-          return SEQVP_OK;
-        }
-      }
+  // NONFINITE_STATE (status uncertain), NONZERO_BOUNDARY, sampling domain (feasibility.py:413-431). Each z plane is
+  // checked for all three on its own thread; the first kind found anywhere, in that order, decides, as three
+  // whole-map passes would.
   {
     const double* axes[3] = {ax, ay, az};
     const int64_t len[3] = {nx, ny, nz};
-    bool bad = false;
-    for (int64_t z = 0; z < nz && !bad; ++z)
-      for (int64_t y = 0; y < ny && !bad; ++y)
-        for (int64_t x = 0; x < nx && !bad; ++x) {
+    std::vector<uint8_t> found((size_t)(3 * nz), 0);
+    host::parallel_for(nz, workers, [&](int64_t z) {
+      bool nonfinite = false, boundary = false, domain = false;
+      for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x) {
           const int64_t idx[3] = {x, y, z};
-          const bool interior = z > 0 && z < nz - 1 && y > 0 && y < ny - 1 && x > 0 && x < nx - 1;
+          const bool face = z == 0 || z == nz - 1 || y == 0 || y == ny - 1 || x == 0 || x == nx - 1;
           const double* p = u + 3 * ((z * ny + y) * nx + x);
           for (int j = 0; j < 3; ++j) {
+            if (!iv::is_finite(p[j])) nonfinite = true;
+            if (face && p[j] != 0.0) boundary = true;
             const double* A = axes[j];
             const double v = p[j] + A[idx[j]];
             const double lo = A[0], hi = A[len[j] - 1];
-            if (!iv::is_finite(v) || v < lo || v > hi || (interior && (v <= lo || v >= hi))) bad = true;
+            if (!iv::is_finite(v) || v < lo || v > hi || (!face && (v <= lo || v >= hi))) domain = true;
           }
         }
-    if (bad) {
+      found[(size_t)(3 * z)] = nonfinite;
+      found[(size_t)(3 * z + 1)] = boundary;
+      found[(size_t)(3 * z + 2)] = domain;
+    });
+    bool any[3] = {false, false, false};
+    for (int64_t z = 0; z < nz; ++z)
+      for (int k = 0; k < 3; ++k) any[k] = any[k] || found[(size_t)(3 * z + k)];
+    if (any[0]) {
+      host::early(out, host::UNCERTAIN, SEQVP_CERT_NONFINITE_STATE, d.ncell);
+      return SEQVP_OK;
+    }
+    if (any[1]) {
+      host::early(out, host::INFEASIBLE, SEQVP_CERT_NONZERO_BOUNDARY, d.ncell);
+      return SEQVP_OK;
+    }
+    if (any[2]) {
       host::early(out, host::INFEASIBLE, SEQVP_CERT_INVALID_DOMAIN, d.ncell);
       return SEQVP_OK;
     }
   }
-  // 1. edge-hull margins (parallel, per-cell independent) and the tile rule
+  // 1. edge-hull margins (parallel, per-cell independent) and the tile rule: a tile with a flagged cell leaves all
+  // its cells to the later stages, otherwise its cells with a positive margin are proven. Per tile in parallel;
+  // the minimum of the proven (finite, positive) margins does not depend on the order it is taken in.
   std::vector<double> margin((size_t)d.ncell);
   std::vector<uint8_t> flag((size_t)d.ncell);
   const int64_t chunk = 16384;
-  const int64_t nchunks = (d.ncell + chunk - 1) / chunk;
-  host::parallel_for(nchunks, workers, [&](int64_t k) {
-    double uv[8][3];
-    for (int64_t c = k * chunk; c < std::min(d.ncell, (k + 1) * chunk); ++c) {
-      const int64_t x0 = c % d.cx, y0 = (c / d.cx) % d.cy, z0 = c / (d.cx * d.cy);
-      const int64_t b = (z0 * ny + y0) * nx + x0;
-      for (int a = 0; a < 8; ++a) {
-        const double* p = u + 3 * (b + (a & 1) + ((a >> 1) & 1) * nx + ((a >> 2) & 1) * nx * ny);
-        uv[a][0] = p[0];
-        uv[a][1] = p[1];
-        uv[a][2] = p[2];
-      }
-      bool ok = true;
-      margin[(size_t)c] = iv::edge_hull_margin<true>(uv, spacing, delta, ok);   // unflagged margins: bitwise
-      flag[(size_t)c] = ok ? 0 : 1;
-    }
-  });
+  host::hull_all(u, d, nx, ny, spacing, delta, workers, margin.data(), flag.data());
   host::Totals t;
   t.checked = d.ncell;
-  std::vector<int64_t> rest;
-  for (int64_t s = 0; s < d.ncell; s += tile) {
-    const int64_t e = std::min(d.ncell, s + tile);
+  const int64_t ntiles = (d.ncell + tile - 1) / tile;
+  std::vector<int64_t> proven((size_t)ntiles, 0);
+  std::vector<double> least((size_t)ntiles, 0.0);
+  std::vector<uint8_t> clean((size_t)ntiles, 0);
+  host::parallel_for(ntiles, workers, [&](int64_t k) {
+    const int64_t s0 = k * tile, e = std::min(d.ncell, s0 + tile);
     bool flagged = false;
-    for (int64_t c = s; c < e; ++c) flagged = flagged || flag[(size_t)c];
-    for (int64_t c = s; c < e; ++c) {
-      if (!flagged && margin[(size_t)c] > 0.0) {
-        t.margin_in(margin[(size_t)c]);
-        ++t.fast;
-      } else {
-        rest.push_back(c);
+    for (int64_t c = s0; c < e; ++c) flagged = flagged || flag[(size_t)c];
+    clean[(size_t)k] = !flagged;
+    if (flagged) return;
+    int64_t n = 0;
+    double m = 0.0;
+    for (int64_t c = s0; c < e; ++c)
+      if (margin[(size_t)c] > 0.0) {
+        m = n ? (margin[(size_t)c] < m ? margin[(size_t)c] : m) : margin[(size_t)c];
+        ++n;
       }
+    proven[(size_t)k] = n;
+    least[(size_t)k] = m;
+  });
+  std::vector<int64_t> rest;
+  for (int64_t k = 0; k < ntiles; ++k) {
+    const int64_t s0 = k * tile, e = std::min(d.ncell, s0 + tile);
+    if (proven[(size_t)k]) {
+      t.margin_in(least[(size_t)k]);
+      t.fast += proven[(size_t)k];
     }
+    if (proven[(size_t)k] == e - s0) continue;
+    for (int64_t c = s0; c < e; ++c)
+      if (!(clean[(size_t)k] && margin[(size_t)c] > 0.0)) rest.push_back(c);
   }
   t.boxes = t.fast;
   // 2. prescreen of the cells the hull left (when a derivative table is given)

@@ -151,3 +151,33 @@ def test_map_cases_cover_every_stage():
             seen.add("subdivision")
     assert {"hull", "prescreen", "subdivision", "JACOBIAN_VIOLATION", "arithmetic_uncertain",
             "NONFINITE_STATE", "NONZERO_BOUNDARY", "INVALID_DOMAIN", "positive_enclosure_cover"} <= seen, seen
+
+
+@pytest.mark.parametrize("delta", [0., .01])
+def test_host_edge_hull_is_the_oracle_per_cell(delta):
+    """The host edge hull (four cells per AVX2 evaluation where the CPU has it, the row ends one by one) against
+    the oracle, cell by cell: every unflagged margin bitwise, every flag equal. Rows of 13 cells mix both paths;
+    the field mixes smooth, rough, zero, huge, tiny and nonfinite vertex values."""
+    from seq_vp_3d_amd import _lib
+    rng = np.random.default_rng(11)
+    shape = (9, 11, 14)
+    spacing = np.array([1.1, .7, 1.3])
+    u = smooth_field(shape, .8, 5) + rough_field(shape, .2, 6)
+    flat = u.reshape(-1, 3)
+    pick = rng.choice(flat.shape[0], 60, replace=False)
+    flat[pick[:10]] = 0.
+    flat[pick[10:20]] *= 1e300
+    flat[pick[20:30]] *= 1e-300
+    flat[pick[30:33], 1] = np.nan
+    flat[pick[33:36], 2] = np.inf
+    u = np.ascontiguousarray(u)
+    nz, ny, nx = shape
+    ncell = (nx - 1) * (ny - 1) * (nz - 1)
+    margin = np.empty(ncell)
+    flag = np.empty(ncell, dtype=np.uint8)
+    _lib.call("seqvp_edge_hull_host", _lib.hptr(u), nx, ny, nz, _lib.hptr(spacing), delta, 3, _lib.hptr(margin),
+              _lib.hptr(flag))
+    want, bad = oc.edge_hull_margins(oc.cell_vertices(u, shape), spacing, delta)
+    assert np.array_equal(flag.astype(bool), bad)
+    assert bad.any() and (~bad).sum() > ncell // 2
+    np.testing.assert_array_equal(_bits(margin[~bad]), _bits(want[~bad]))

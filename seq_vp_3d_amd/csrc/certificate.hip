@@ -138,10 +138,11 @@ __global__ void __launch_bounds__(CB) decide_kernel(const double* __restrict__ u
       f += snf[k];
       p += snp[k];
     }
-    part[2 * blk] = a;
-    part[2 * blk + 1] = b;
-    if (f) atomicAdd(counters, (unsigned long long)f);
-    if (p) atomicAdd(counters + 1, (unsigned long long)p);
+    // the counts go to the block's slots too (one atomic per block on a shared counter serialises 26k blocks)
+    part[4 * blk] = a;
+    part[4 * blk + 1] = b;
+    part[4 * blk + 2] = (double)f;
+    part[4 * blk + 3] = (double)p;
   }
 }
 
@@ -149,31 +150,40 @@ __global__ void __launch_bounds__(CB) decide_kernel(const double* __restrict__ u
 __global__ void __launch_bounds__(CB) screen_finish_kernel(const double* __restrict__ part, int64_t nblocks,
                                                            const unsigned long long* __restrict__ counters,
                                                            double* __restrict__ summary) {
-  double a = __builtin_inf(), b = __builtin_inf();
+  // minima exact in any order; the counts are integers below 2^53, so their double sums are exact too
+  double a = __builtin_inf(), b = __builtin_inf(), f = 0.0, p = 0.0;
   for (int64_t i = threadIdx.x; i < nblocks; i += CB) {
-    a = fmin(a, part[2 * i]);
-    b = fmin(b, part[2 * i + 1]);
+    a = fmin(a, part[4 * i]);
+    b = fmin(b, part[4 * i + 1]);
+    f += part[4 * i + 2];
+    p += part[4 * i + 3];
   }
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
     a = fmin(a, __shfl_xor(a, o, 64));
     b = fmin(b, __shfl_xor(b, o, 64));
+    f += __shfl_xor(f, o, 64);
+    p += __shfl_xor(p, o, 64);
   }
-  __shared__ double sa[CB / 64], sb[CB / 64];
+  __shared__ double sa[CB / 64], sb[CB / 64], sf[CB / 64], sp[CB / 64];
   if ((threadIdx.x & 63) == 0) {
     sa[threadIdx.x >> 6] = a;
     sb[threadIdx.x >> 6] = b;
+    sf[threadIdx.x >> 6] = f;
+    sp[threadIdx.x >> 6] = p;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k = 1; k < CB / 64; ++k) {
       a = fmin(a, sa[k]);
       b = fmin(b, sb[k]);
+      f += sf[k];
+      p += sp[k];
     }
     summary[0] = a;
     summary[1] = b;
-    summary[2] = (double)counters[0];
-    summary[3] = (double)counters[1];
+    summary[2] = f;
+    summary[3] = p;
     summary[4] = (double)counters[2];
   }
 }
@@ -763,7 +773,7 @@ extern "C" size_t seqvp_certificate_workspace_bytes(int64_t nx, int64_t ny, int6
   const Dims d = make_dims(nx, ny, nz);
   const int64_t nb = (d.ncell + CB - 1) / CB;
   const int64_t ntile = (d.ncell + tile - 1) / tile;
-  return (size_t)(8 * d.ncell + 16 * nb + 32 + ((ntile + 255) / 256) * 256);
+  return (size_t)(8 * d.ncell + 32 * nb + 32 + ((ntile + 255) / 256) * 256);
 }
 
 extern "C" int seqvp_edge_hull(const double* u, double hx, double hy, double hz, double delta, int64_t nx, int64_t ny,
@@ -788,8 +798,8 @@ extern "C" int seqvp_certificate_screen(const double* u, const double* D, double
   char* ws = (char*)workspace;
   double* margin = (double*)ws;
   double* part = (double*)(ws + 8 * d.ncell);
-  unsigned long long* counters = (unsigned long long*)(ws + 8 * d.ncell + 16 * nb);
-  uint8_t* tile_flag = (uint8_t*)(ws + 8 * d.ncell + 16 * nb + 32);
+  unsigned long long* counters = (unsigned long long*)(ws + 8 * d.ncell + 32 * nb);
+  uint8_t* tile_flag = (uint8_t*)(ws + 8 * d.ncell + 32 * nb + 32);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(zero_bytes_kernel, dim3((unsigned)std::min<int64_t>((ntile + CB - 1) / CB, 1024)), dim3(CB), 0, s,
                      tile_flag, ntile);

@@ -152,11 +152,23 @@ __global__ void __launch_bounds__(CB) screen_finish_kernel(const double* __restr
                                                            double* __restrict__ summary) {
   // minima exact in any order; the counts are integers below 2^53, so their double sums are exact too
   double a = __builtin_inf(), b = __builtin_inf(), f = 0.0, p = 0.0;
-  for (int64_t i = threadIdx.x; i < nblocks; i += CB) {
-    a = fmin(a, part[4 * i]);
-    b = fmin(b, part[4 * i + 1]);
-    f += part[4 * i + 2];
-    p += part[4 * i + 3];
+  constexpr int BATCH = 8;   // loads of eight blocks in flight at once; the fold order is unchanged
+  for (int64_t i0 = threadIdx.x; i0 < nblocks; i0 += BATCH * CB) {
+    double v[BATCH][4];
+#pragma unroll
+    for (int k = 0; k < BATCH; ++k) {
+      const int64_t i = i0 + k * CB < nblocks ? i0 + k * CB : nblocks - 1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[k][q] = part[4 * i + q];
+    }
+#pragma unroll
+    for (int k = 0; k < BATCH; ++k) {
+      const bool in = i0 + k * CB < nblocks;   // selected, not branched (keeps the loads ahead)
+      a = in ? fmin(a, v[k][0]) : a;
+      b = in ? fmin(b, v[k][1]) : b;
+      f = in ? f + v[k][2] : f;
+      p = in ? p + v[k][3] : p;
+    }
   }
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {

@@ -61,12 +61,32 @@ __global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __
   for (int q = 0; q < NS; ++q) s[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = __builtin_inf();
-  for (int64_t b = threadIdx.x; b < nblocks; b += BLOCK) {
-    const double* p = partials + b * STRIDE;
+  // Each thread folds blocks t, t + BLOCK, t + 2 BLOCK, ... in that order; eight of them are loaded at a time
+  // (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop.
+  constexpr int BATCH = 8;
+  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * BLOCK) {
+    double v[BATCH][STRIDE];
 #pragma unroll
-    for (int q = 0; q < NS; ++q) dd_add(s[q], DD{p[2 * q], p[2 * q + 1]});
+    for (int i = 0; i < BATCH; ++i) {
+      const int64_t b = b0 + i * BLOCK < nblocks ? b0 + i * BLOCK : nblocks - 1;
 #pragma unroll
-    for (int q = 0; q < NM; ++q) m[q] = nan_min(m[q], p[2 * NS + q]);
+      for (int q = 0; q < STRIDE; ++q) v[i][q] = partials[b * STRIDE + q];
+    }
+    // a block past the end leaves the accumulators as they are (selected, not branched, so that the compiler
+    // keeps the eight loads ahead of the arithmetic)
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const bool in = b0 + i * BLOCK < nblocks;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        DD t = s[q];
+        dd_add(t, DD{v[i][2 * q], v[i][2 * q + 1]});
+        s[q].hi = in ? t.hi : s[q].hi;
+        s[q].lo = in ? t.lo : s[q].lo;
+      }
+#pragma unroll
+      for (int q = 0; q < NM; ++q) m[q] = in ? nan_min(m[q], v[i][2 * NS + q]) : m[q];
+    }
   }
   __shared__ double fin[STRIDE];
   block_reduce_store<NS, NM>(s, m, fin);
@@ -998,10 +1018,22 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
 __global__ void __launch_bounds__(BLOCK) jd_root_reduce_kernel(const double* __restrict__ partials, int64_t nblocks,
                                                                double* __restrict__ out) {
   double best = __builtin_inf(), invalid = 0.0, bad = 0.0;
-  for (int64_t b = threadIdx.x; b < nblocks; b += BLOCK) {
-    best = fmin(best, partials[3 * b]);
-    invalid += partials[3 * b + 1];
-    bad = fmax(bad, partials[3 * b + 2]);
+  constexpr int BATCH = 8;   // loads of eight blocks in flight at once; the fold order is unchanged
+  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * BLOCK) {
+    double v[BATCH][3];
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const int64_t b = b0 + i * BLOCK < nblocks ? b0 + i * BLOCK : nblocks - 1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[i][q] = partials[3 * b + q];
+    }
+#pragma unroll
+    for (int i = 0; i < BATCH; ++i) {
+      const bool in = b0 + i * BLOCK < nblocks;   // selected, not branched (keeps the loads ahead)
+      best = in ? fmin(best, v[i][0]) : best;
+      invalid = in ? invalid + v[i][1] : invalid;
+      bad = in ? fmax(bad, v[i][2]) : bad;
+    }
   }
   __shared__ double sh[BLOCK / 64][3];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

@@ -168,7 +168,8 @@ def hptr(a):
         return None
     if not a.flags.c_contiguous:
         raise ValueError("C-contiguous host array expected")
-    return ctypes.c_void_p(a.ctypes.data)
+    # the array interface's address (a.ctypes builds a helper object per call, ~10 us)
+    return ctypes.c_void_p(a.__array_interface__["data"][0])
 
 
 def ptr(t):

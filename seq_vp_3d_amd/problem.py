@@ -9,6 +9,8 @@ certificate_config.delta, barrier_activation), so a reference Problem3D can be h
 from dataclasses import dataclass
 from itertools import product
 
+import math
+
 import numpy as np
 
 VERTICES = tuple((x, y, z) for z, y, x in product((0, 1), repeat=3))
@@ -30,7 +32,7 @@ class Grid3D:
 
     @property
     def omega(self):
-        return float(np.prod(self.spacing))
+        return float(math.prod(self.spacing))        # ((h0 h1) h2), the same product as np.prod
 
     @property
     def interior_shape(self):
@@ -42,15 +44,15 @@ class Grid3D:
 
     @property
     def n_cells(self):
-        return int(np.prod(self.cell_shape))
+        return math.prod(self.cell_shape)
 
     @property
     def n_nodes(self):
-        return int(np.prod(self.shape))
+        return math.prod(self.shape)
 
     @property
     def n_free_scalar(self):
-        return int(np.prod(self.interior_shape))
+        return math.prod(self.interior_shape)
 
     @property
     def axes(self):

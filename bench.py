@@ -25,6 +25,7 @@ FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 14445.,
     "seqvp_q1_barrier_energy": 8460.,
     "seqvp_q1_gradient": 12971. + 8093.,
+    "seqvp_q1_gradient_free": 12971. + 8093.,
     "seqvp_q1_ggn_diagonal": 27290.,
     "seqvp_jd_root": 4015.,
     "seqvp_spline_warp": 1078.,
@@ -36,6 +37,7 @@ EXECUTED_FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 8411.,
     "seqvp_q1_barrier_energy": 4603.,
     "seqvp_q1_gradient": 10197. + 24.,
+    "seqvp_q1_gradient_free": 10197. + 24.,
     "seqvp_q1_ggn_diagonal": 14167.,
     "seqvp_jd_root": 6536.,
     "seqvp_spline_warp": 967.,
@@ -48,6 +50,7 @@ TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_soft_energy": 986e6,
     "seqvp_q1_barrier_energy": 362e6,
     "seqvp_q1_gradient": 3666e6,
+    "seqvp_q1_gradient_free": 4344e6,
     "seqvp_q1_joint": 6353e6,
     "seqvp_jd_root": 361e6,
     "seqvp_certificate_screen": 283e6,

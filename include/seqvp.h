@@ -88,6 +88,15 @@ int seqvp_q1_gradient(const double* node_base, const double* u, const double* re
                       const double* k, int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* The energy gradient on the free nodes in one call (replaces TorchLinearization.gradient followed by the
+ * free-node restriction, torch_ops.py:559-568; linearization.py:150-201): g = restrict(node terms total + Gauss
+ * JD/curl VJP + closed27 barrier VJP), g component-major (3, nz-2, ny-2, nx-2). The node terms are those of
+ * seqvp_node_terms and the gather order that of seqvp_q1_gradient; the node field is never written. */
+int seqvp_q1_gradient_free(const double* warped, const double* fixed, const double* data_weight,
+                           const double* image_grad, const double* u, const double* ref, double lam, double hx,
+                           double hy, double hz, const double* k, int64_t nx, int64_t ny, int64_t nz, double* g,
+                           void* workspace, size_t workspace_bytes, void* stream);
+
 /* GF and GE node gradients sharing the cell work (replaces FusedQ1.joint, triton_q1.py:563-570;
  * linearization.py:217-238): outer = outer_base + barrier VJP; total = total_base + soft VJP +
  * barrier VJP. */

@@ -1107,6 +1107,41 @@ __global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __rest
   block_reduce_store<4, 0>(s, none, partials + (int64_t)blockIdx.x * 8);
 }
 
+// The energy gradient on the free nodes in one pass: for interior node f (trapezoid weight 1, so q = omega) the
+// total node term qd (W - F) b + lam q (u - ref), written as node_terms_kernel writes it, plus the eight cell
+// contributions in vertex order a = 0..7 as gather_kernel adds them, stored straight into the component-major free
+// vector g[c F + f] (what restrict_kernel would make of the node field). Saves the node-term and restriction passes
+// over the whole node field.
+__global__ void __launch_bounds__(BLOCK) gather_free_kernel(const double* __restrict__ warped,
+                                                            const double* __restrict__ fixed,
+                                                            const double* __restrict__ dw,
+                                                            const double* __restrict__ bgrad,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ ref, double lam, double omega,
+                                                            const double* __restrict__ loc, Dims d,
+                                                            double* __restrict__ g) {
+  const int64_t mx = d.nx - 2, my = d.ny - 2, mz = d.nz - 2, F = mx * my * mz;
+  const int64_t f = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (f >= F) return;
+  const int64_t x = f % mx + 1, y = (f / mx) % my + 1, z = f / (mx * my) + 1;
+  const int64_t i = (z * d.ny + y) * d.nx + x;
+  const double q = omega * 1.0 * 1.0 * 1.0;
+  const double qd = dw ? q * dw[i] : q;
+  const double lq = lam * q;
+  const double r = qd * (warped[i] - fixed[i]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double im = r * bgrad[3 * i + c];
+    double acc = im + lq * (u[3 * i + c] - ref[3 * i + c]);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int64_t cell = ((z - ((a >> 2) & 1)) * d.cy + (y - ((a >> 1) & 1))) * d.cx + (x - (a & 1));
+      acc += loc[(int64_t)(a * 3 + c) * d.ncell + cell];
+    }
+    g[c * F + f] = acc;
+  }
+}
+
 // Node terms of the gradient and of the GGN diagonal (torch_ops.py:563-568, 784-791;
 // linearization.py:150-156, 362): image = qd (W - F) b; total = image + lam q (u - ref);
 // ggn = qd b^2 + lam q. Any output may be NULL.
@@ -1225,6 +1260,24 @@ extern "C" int seqvp_q1_gradient(const double* node_base, const double* u, const
     hipLaunchKernelGGL((grad_locals_kernel<1, false, true>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, loc, loc);
   hipLaunchKernelGGL((gather_kernel<false>), dim3(cdiv(nx * ny * nz, BLOCK)), dim3(BLOCK), 0, s, node_base, loc,
                      loc, d, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_gradient_free(const double* warped, const double* fixed, const double* data_weight,
+                                      const double* image_grad, const double* u, const double* ref, double lam,
+                                      double hx, double hy, double hz, const double* k, int64_t nx, int64_t ny,
+                                      int64_t nz, double* g, void* ws, size_t ws_bytes, void* stream) {
+  if (!warped || !fixed || !image_grad || !u || !ref || !k || !g || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(2, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  const int64_t F = (nx - 2) * (ny - 2) * (nz - 2);
+  hipStream_t s = (hipStream_t)stream;
+  double* loc = (double*)ws;
+  hipLaunchKernelGGL((grad_locals_kernel<0, true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, k, d, loc, loc);
+  if (F > 0)
+    hipLaunchKernelGGL(gather_free_kernel, dim3(cdiv(F, BLOCK)), dim3(BLOCK), 0, s, warped, fixed, data_weight,
+                       image_grad, u, ref, lam, hx * hy * hz, loc, d, g);
   return launch_status();
 }
 

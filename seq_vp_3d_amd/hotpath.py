@@ -233,10 +233,8 @@ class PairState:
     def gradient(self):
         """Full energy gradient on theta (device), g = restrict(node terms + soft + barrier VJPs)."""
         q, w = self.q1, self.problem.weights
-        q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
-                     total=self.node_a)
-        q.gradient_nodes(self.node_a, self.u, self.ref, out=self.node_b)
-        return self.restrict(self.node_b, self.g)
+        return q.gradient_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam,
+                               data_weight=self.data_weight, out=self.g)
 
     def direction(self, fp32=False):
         """p = -S M^-1 (S g)."""

@@ -115,6 +115,25 @@ class FusedQ1:
                   int(barrier), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
+    def gradient_free(self, warped, fixed, image_grad, u, ref, lam, data_weight=None, out=None):
+        """restrict(node terms total + Gauss soft VJP + closed27 barrier VJP) in one call: the energy gradient on
+        the free nodes, component-major (3 nz-2 ny-2 nx-2 values)."""
+        self._check(u, ref, image_grad)
+        for a in (warped, fixed) + (() if data_weight is None else (data_weight,)):
+            if a.dtype != torch.float64 or tuple(a.shape) != self.shape or not a.is_contiguous():
+                raise InputValidationError("node scalar fields must be contiguous float64 (nz, ny, nx)")
+        nz, ny, nx = self.shape
+        F = (nx - 2) * (ny - 2) * (nz - 2)
+        out = torch.empty(3 * F, dtype=torch.float64, device=self.device) if out is None else out
+        if out.dtype != torch.float64 or out.numel() != 3 * F or not out.is_contiguous():
+            raise InputValidationError("out must be a contiguous float64 vector of 3 (nz-2)(ny-2)(nx-2) values")
+        w, n = self._workspace(OP_GRAD)
+        hx, hy, hz = self.grid.spacing
+        _lib.call("seqvp_q1_gradient_free", _lib.ptr(warped), _lib.ptr(fixed), _lib.ptr(data_weight),
+                  _lib.ptr(image_grad), _lib.ptr(u), _lib.ptr(ref), float(lam), hx, hy, hz, _lib.ptr(self.k),
+                  *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
     def joint(self, image, total_base, u, ref):
         """(outer = image + barrier, total = total_base + soft + barrier) sharing the cell work
         (triton_q1.py:563-570)."""

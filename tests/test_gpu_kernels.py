@@ -304,6 +304,32 @@ def test_node_terms(gpu_device):
     np.testing.assert_allclose(ggn.cpu().numpy(), m.ggn_diagonal_base(b), rtol=1e-14)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_gradient_free_is_the_restricted_node_gradient(gpu_device, weighted):
+    """seqvp_q1_gradient_free (node terms, VJP gather and restriction in one pass) against the oracle's node
+    gradient restricted to the free nodes, and against the three-call device path."""
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case()
+    ref, _, u = fields(og.shape, 130)
+    rng = np.random.default_rng(131)
+    fixed, warped = rng.uniform(size=og.shape), rng.uniform(size=og.shape)
+    b = rng.normal(size=og.shape + (3,))
+    dw = rng.uniform(.5, 1., size=og.shape) if weighted else None
+    m.fixed = fixed
+    m.qd = m.q * dw if weighted else m.q
+    q = FusedQ1(prob, gpu_device)
+    D = lambda a: None if a is None else dev(a, gpu_device)
+    print("case gradient_free", og.shape, "weighted", weighted, flush=True)
+    g = q.gradient_free(D(warped), D(fixed), D(b), D(u), D(ref), m.lam, data_weight=D(dw)).cpu().numpy()
+    want = Q.restrict_free(m.gradient_nodes(m.node_gradient_base(u, ref, warped, b), u, ref), og)
+    np.testing.assert_allclose(g, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+    tot = torch.empty(og.shape + (3,), dtype=torch.float64, device=gpu_device)
+    q.node_terms(D(warped), D(fixed), D(b), D(u), D(ref), m.lam, data_weight=D(dw), total=tot)
+    nodes = q.gradient_nodes(tot, D(u), D(ref)).cpu().numpy()
+    three = Q.restrict_free(nodes, og)
+    np.testing.assert_allclose(g, three, rtol=1e-14, atol=1e-15 * np.abs(three).max())
+
+
 @pytest.mark.parametrize("activation", [.05, .9])
 def test_fp32_screen_energies_within_band(gpu_device, activation):
     """The FP32 screening energies stay within the screen's band of the FP64 values

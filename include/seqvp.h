@@ -97,6 +97,15 @@ int seqvp_q1_gradient_free(const double* warped, const double* fixed, const doub
                            double hy, double hz, const double* k, int64_t nx, int64_t ny, int64_t nz, double* g,
                            void* workspace, size_t workspace_bytes, void* stream);
 
+/* The outer gradients on the free nodes in one call (replaces the node terms, FusedQ1.joint and the three
+ * restrictions of the outer norms, linearization.py:217-238; sequential.py:288-338): gd = restrict(image term),
+ * gf = restrict(image term + barrier VJP), ge = restrict(total term + Gauss JD/curl VJP + barrier VJP), each
+ * component-major (3, nz-2, ny-2, nx-2), the gather order that of seqvp_q1_joint. */
+int seqvp_q1_joint_free(const double* warped, const double* fixed, const double* data_weight, const double* image_grad,
+                        const double* u, const double* ref, double lam, double hx, double hy, double hz,
+                        const double* k, int64_t nx, int64_t ny, int64_t nz, double* gd, double* gf, double* ge,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
 /* GF and GE node gradients sharing the cell work (replaces FusedQ1.joint, triton_q1.py:563-570;
  * linearization.py:217-238): outer = outer_base + barrier VJP; total = total_base + soft VJP +
  * barrier VJP. */

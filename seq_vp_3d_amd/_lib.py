@@ -27,6 +27,8 @@ SIGNATURES = {
     "seqvp_q1_gradient": [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_gradient_free": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_i64, c_i64,
                                c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_joint_free": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_i64,
+                            c_vp, c_vp, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_joint": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_ggn_diagonal": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_node_min_jd": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],

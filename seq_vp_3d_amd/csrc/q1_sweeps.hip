@@ -1142,6 +1142,49 @@ __global__ void __launch_bounds__(BLOCK) gather_free_kernel(const double* __rest
   }
 }
 
+// The three outer gradients on the free nodes in one pass (the joint form): gd = image node term, gf = image +
+// barrier contributions, ge = total node term + soft contributions + barrier contributions, each gathered in vertex
+// order as gather_kernel does (soft before barrier for ge) and stored component-major.
+__global__ void __launch_bounds__(BLOCK) joint_free_kernel(const double* __restrict__ warped,
+                                                           const double* __restrict__ fixed,
+                                                           const double* __restrict__ dw,
+                                                           const double* __restrict__ bgrad,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ ref, double lam, double omega,
+                                                           const double* __restrict__ soft,
+                                                           const double* __restrict__ bar, Dims d,
+                                                           double* __restrict__ gd, double* __restrict__ gf,
+                                                           double* __restrict__ ge) {
+  const int64_t mx = d.nx - 2, my = d.ny - 2, mz = d.nz - 2, F = mx * my * mz;
+  const int64_t f = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (f >= F) return;
+  const int64_t x = f % mx + 1, y = (f / mx) % my + 1, z = f / (mx * my) + 1;
+  const int64_t i = (z * d.ny + y) * d.nx + x;
+  const double q = omega * 1.0 * 1.0 * 1.0;
+  const double qd = dw ? q * dw[i] : q;
+  const double lq = lam * q;
+  const double r = qd * (warped[i] - fixed[i]);
+  int64_t cells[8];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+    cells[a] = ((z - ((a >> 2) & 1)) * d.cy + (y - ((a >> 1) & 1))) * d.cx + (x - (a & 1));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double im = r * bgrad[3 * i + c];
+    double outer = im;
+    double total = im + lq * (u[3 * i + c] - ref[3 * i + c]);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) outer += bar[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) total += soft[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) total += bar[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
+    gd[c * F + f] = im;
+    gf[c * F + f] = outer;
+    ge[c * F + f] = total;
+  }
+}
+
 // Node terms of the gradient and of the GGN diagonal (torch_ops.py:563-568, 784-791;
 // linearization.py:150-156, 362): image = qd (W - F) b; total = image + lam q (u - ref);
 // ggn = qd b^2 + lam q. Any output may be NULL.
@@ -1295,6 +1338,26 @@ extern "C" int seqvp_q1_joint(const double* outer_base, const double* total_base
   const int64_t ng = cdiv(nx * ny * nz, BLOCK);
   hipLaunchKernelGGL((gather_kernel<false>), dim3(ng), dim3(BLOCK), 0, s, outer_base, bar, bar, d, outer);
   hipLaunchKernelGGL((gather_kernel<true>), dim3(ng), dim3(BLOCK), 0, s, total_base, soft, bar, d, total);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_joint_free(const double* warped, const double* fixed, const double* data_weight,
+                                   const double* image_grad, const double* u, const double* ref, double lam, double hx,
+                                   double hy, double hz, const double* k, int64_t nx, int64_t ny, int64_t nz,
+                                   double* gd, double* gf, double* ge, void* ws, size_t ws_bytes, void* stream) {
+  if (!warped || !fixed || !image_grad || !u || !ref || !k || !gd || !gf || !ge || !grid_ok(nx, ny, nz))
+    return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(4, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  const int64_t F = (nx - 2) * (ny - 2) * (nz - 2);
+  hipStream_t s = (hipStream_t)stream;
+  double* soft = (double*)ws;
+  double* bar = soft + 24 * d.ncell;
+  hipLaunchKernelGGL((grad_locals_kernel<2, true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, k, d, soft, bar);
+  if (F > 0)
+    hipLaunchKernelGGL(joint_free_kernel, dim3(cdiv(F, BLOCK)), dim3(BLOCK), 0, s, warped, fixed, data_weight,
+                       image_grad, u, ref, lam, hx * hy * hz, soft, bar, d, gd, gf, ge);
   return launch_status();
 }
 

@@ -369,12 +369,9 @@ class PairState:
         """(GF, GD, GE) = mass norms of the outer (image + barrier), image-only and total gradients
         (sequential.py:288-338; linearization.py:217-238)."""
         q, w = self.q1, self.problem.weights
-        q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
-                     image=self.node_a, total=self.node_b)
-        outer, total = q.joint(self.node_a, self.node_b, self.u, self.ref)
-        gd = self.restrict(self.node_a, self.p)        # p, diag and g are free here (recomputed before use)
-        gf = self.restrict(outer, self.diag)
-        ge = self.restrict(total, self.g)
+        # p, diag and g are free here (recomputed before use)
+        gd, gf, ge = q.joint_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam,
+                                  data_weight=self.data_weight, out=(self.p, self.diag, self.g))
         ff, dd, ee = self.dots((gf, gf), (gd, gd), (ge, ge))
         r = float(np.sqrt(self.grid.omega))
         return float(np.sqrt(ff)) / r, float(np.sqrt(dd)) / r, float(np.sqrt(ee)) / r

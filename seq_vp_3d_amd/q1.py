@@ -134,6 +134,27 @@ class FusedQ1:
                   *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
+    def joint_free(self, warped, fixed, image_grad, u, ref, lam, data_weight=None, out=None):
+        """(gd, gf, ge) on the free nodes in one call: restrict(image term), restrict(image term + barrier VJP),
+        restrict(total term + soft VJP + barrier VJP). `out` = three contiguous float64 vectors of 3F values."""
+        self._check(u, ref, image_grad)
+        for a in (warped, fixed) + (() if data_weight is None else (data_weight,)):
+            if a.dtype != torch.float64 or tuple(a.shape) != self.shape or not a.is_contiguous():
+                raise InputValidationError("node scalar fields must be contiguous float64 (nz, ny, nx)")
+        nz, ny, nx = self.shape
+        F = (nx - 2) * (ny - 2) * (nz - 2)
+        if out is None:
+            out = tuple(torch.empty(3 * F, dtype=torch.float64, device=self.device) for _ in range(3))
+        for o in out:
+            if o.dtype != torch.float64 or o.numel() != 3 * F or not o.is_contiguous():
+                raise InputValidationError("outputs must be contiguous float64 vectors of 3 (nz-2)(ny-2)(nx-2) values")
+        w, n = self._workspace(OP_JOINT)
+        hx, hy, hz = self.grid.spacing
+        _lib.call("seqvp_q1_joint_free", _lib.ptr(warped), _lib.ptr(fixed), _lib.ptr(data_weight),
+                  _lib.ptr(image_grad), _lib.ptr(u), _lib.ptr(ref), float(lam), hx, hy, hz, _lib.ptr(self.k),
+                  *self._dims(), *(_lib.ptr(o) for o in out), w, n, _lib.stream_handle(self.device))
+        return out
+
     def joint(self, image, total_base, u, ref):
         """(outer = image + barrier, total = total_base + soft + barrier) sharing the cell work
         (triton_q1.py:563-570)."""

@@ -330,6 +330,34 @@ def test_gradient_free_is_the_restricted_node_gradient(gpu_device, weighted):
     np.testing.assert_allclose(g, three, rtol=1e-14, atol=1e-15 * np.abs(three).max())
 
 
+def test_joint_free_is_the_restricted_joint(gpu_device):
+    """seqvp_q1_joint_free (node terms, joint gathers and the three restrictions in one pass) against the oracle's
+    image, outer and total node gradients restricted to the free nodes, and against the multi-call device path."""
+    from seq_vp_3d_amd.q1 import FusedQ1
+    prob, og, m = case(activation=.9)
+    ref, _, u = fields(og.shape, 140)
+    rng = np.random.default_rng(141)
+    fixed, warped = rng.uniform(size=og.shape), rng.uniform(size=og.shape)
+    b = rng.normal(size=og.shape + (3,))
+    m.fixed = fixed
+    m.qd = m.q
+    q = FusedQ1(prob, gpu_device)
+    D = lambda a: dev(a, gpu_device)
+    print("case joint_free", og.shape, flush=True)
+    gd, gf, ge = (t.cpu().numpy() for t in q.joint_free(D(warped), D(fixed), D(b), D(u), D(ref), m.lam))
+    img = m.node_gradient_base(u, ref, warped, b, total=False)
+    tot = m.node_gradient_base(u, ref, warped, b)
+    R = lambda a: Q.restrict_free(a, og)
+    for got, want in ((gd, R(img)), (gf, R(m.gradient_nodes(img, u, ref, soft=False, barrier=True))),
+                      (ge, R(m.gradient_nodes(tot, u, ref, soft=True, barrier=True)))):
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+    im_d, tot_d = (torch.empty(og.shape + (3,), dtype=torch.float64, device=gpu_device) for _ in range(2))
+    q.node_terms(D(warped), D(fixed), D(b), D(u), D(ref), m.lam, image=im_d, total=tot_d)
+    outer, total = q.joint(im_d, tot_d, D(u), D(ref))
+    for got, want in ((gd, R(im_d.cpu().numpy())), (gf, R(outer.cpu().numpy())), (ge, R(total.cpu().numpy()))):
+        np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-15 * np.abs(want).max())
+
+
 @pytest.mark.parametrize("activation", [.05, .9])
 def test_fp32_screen_energies_within_band(gpu_device, activation):
     """The FP32 screening energies stay within the screen's band of the FP64 values

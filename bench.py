@@ -19,8 +19,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# algorithmic FP64 operations per cell (per point for the sampler) of each entry point, counted on
-# the reference's generated kernels
+# algorithmic FP64 operations per cell (per point for the sampler) of each entry point: the + - * (an FMA as two)
+# of the reference's generated Triton kernels for the same work
 FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 14445.,
     "seqvp_q1_barrier_energy": 8460.,

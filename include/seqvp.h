@@ -164,6 +164,11 @@ int seqvp_witness_coefficients(const double* u, const double* v, const double* k
 int seqvp_jd_root(const double* u, const double* v, const double* k, double limit, int64_t nx, int64_t ny,
                   int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* seqvp_jd_root with the direction given by its free-node vector p (component-major, 3 x (nx-2)(ny-2)(nz-2)):
+ * the same root, bit for bit, as seqvp_jd_root on inject(p), without writing the node field. */
+int seqvp_jd_root_free(const double* u, const double* p, const double* k, double limit, int64_t nx, int64_t ny,
+                       int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* T75 Bernstein floating-point prescreen (replaces prescreen_lower_bounds,
  * jd_curl/three_d/certificate_prescreen.py:125-153; kernel :60-121), bitwise in IEEE binary64 with no
  * contraction and the reference operation order. D: closed27 derivative table (3,27,8) (device).

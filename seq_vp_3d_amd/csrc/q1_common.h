@@ -77,6 +77,22 @@ __device__ __forceinline__ void load_cell(const double* __restrict__ f, int64_t 
   }
 }
 
+// The 8 vertex values of cell c of a field given by its free-node vector (component-major, 3 x (nx-2)(ny-2)(nz-2),
+// boundary nodes zero): the values load_cell reads from inject(p), without the node field.
+__device__ __forceinline__ void load_cell_free(const double* __restrict__ p, int64_t c, const Dims& d,
+                                               double v[8][3]) {
+  const int64_t mx = d.nx - 2, my = d.ny - 2, F = mx * my * (d.nz - 2);
+  const int64_t x0 = c % d.cx, y0 = (c / d.cx) % d.cy, z0 = c / (d.cx * d.cy);
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int64_t x = x0 + (a & 1), y = y0 + ((a >> 1) & 1), z = z0 + ((a >> 2) & 1);
+    const bool inner = x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
+    const int64_t f = inner ? ((z - 1) * my + (y - 1)) * mx + (x - 1) : 0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) v[a][r] = inner ? p[r * F + f] : 0.0;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void make_edges(const T v[8][3], const T ih[3], EdgesT<T>& E) {
 #pragma unroll

@@ -932,6 +932,8 @@ __device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3]
 }
 
 // Per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129; torch_ops.py:636-673).
+// FREE: v is the direction's free-node vector (the node values of inject(v), read without the node field).
+template <bool FREE>
 __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict__ u, const double* __restrict__ v,
                                                         const double* __restrict__ k, double L, Dims d,
                                                         double* __restrict__ partials) {
@@ -950,7 +952,10 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
       double w[8][3];
       load_cell(u, b, d, w);
       make_edges(w, ih, Eu);
-      load_cell(v, b, d, w);
+      if (FREE)
+        load_cell_free(v, c, d, w);
+      else
+        load_cell(v, b, d, w);
       make_edges(w, ih, Ev);
     }
     for (int p = 0; p < 27; ++p) {
@@ -1462,7 +1467,20 @@ extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, 
   const int64_t nb = cdiv(d.ncell, BLOCK);
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)ws;
-  hipLaunchKernelGGL(jd_root_kernel, dim3(nb), dim3(BLOCK), 0, s, u, v, k, limit, d, part);
+  hipLaunchKernelGGL((jd_root_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, v, k, limit, d, part);
+  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_jd_root_free(const double* u, const double* p, const double* k, double limit, int64_t nx,
+                                  int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!u || !p || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL((jd_root_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, p, k, limit, d, part);
   hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, nb, out);
   return launch_status();
 }

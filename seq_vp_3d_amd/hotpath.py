@@ -241,8 +241,7 @@ class PairState:
         return self.poisson.apply_equilibrated(self.scale, self.g, out=self.p, fp32=fp32)
 
     def jd_bound(self, limit):
-        self.inject(self.p, self.v)
-        return self.root(self.u, self.v, limit)
+        return self.root.from_free(self.u, self.p, limit)
 
     def witness_coefficients(self, cells, points):
         """(n, 4) host array: det(A + s V) - delta at reference points of cells, A = I + grad u of the accepted
@@ -404,10 +403,14 @@ class PairState:
         out = self.scalars
         g, p = self.g, self.p
         xs, ys = (ctypes.c_void_p * 1)(g.data_ptr()), (ctypes.c_void_p * 1)(p.data_ptr())
-        _lib.call("seqvp_dots", xs, ys, 1, g.numel(), _lib.ptr(out[0:1]), _lib.ptr(self.dot_ws), self.dot_ws.numel(),
-                  self._stream())
         if norm:
-            _lib.call("seqvp_dots", xs, xs, 1, g.numel(), _lib.ptr(out[1:2]), _lib.ptr(self.dot_ws),
+            # g.p and g.g in one pass (one launch, g read once from HBM)
+            xs2 = (ctypes.c_void_p * 2)(g.data_ptr(), g.data_ptr())
+            ys2 = (ctypes.c_void_p * 2)(p.data_ptr(), g.data_ptr())
+            _lib.call("seqvp_dots", xs2, ys2, 2, g.numel(), _lib.ptr(out[0:2]), _lib.ptr(self.dot_ws),
+                      self.dot_ws.numel(), self._stream())
+        else:
+            _lib.call("seqvp_dots", xs, ys, 1, g.numel(), _lib.ptr(out[0:1]), _lib.ptr(self.dot_ws),
                       self.dot_ws.numel(), self._stream())
         if prev is not None:
             _lib.call("seqvp_bb2_products", _lib.ptr(self.theta), _lib.ptr(prev["theta"]), _lib.ptr(self.g),

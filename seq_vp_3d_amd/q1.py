@@ -228,3 +228,22 @@ class FusedJDRoot:
         if bad:
             raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
         return float(root), int(invalid), 27 * q1.ncell
+
+    def from_free(self, u, p, limit):
+        """The same bound with the direction as its free-node vector p (no node field)."""
+        q1 = self.q1
+        q1._check(u)
+        nz, ny, nx = q1.shape
+        if (p.dtype != torch.float64 or not p.is_contiguous() or p.device != q1.device
+                or p.numel() != 3 * (nx - 2) * (ny - 2) * (nz - 2)):
+            raise InputValidationError("p must be the contiguous float64 free-node vector")
+        if not (np.isfinite(limit) and limit > 0):
+            raise InputValidationError("limit must be positive and finite")
+        out = torch.empty(3, dtype=torch.float64, device=q1.device)
+        w, n = q1._workspace(OP_ROOT)
+        _lib.call("seqvp_jd_root_free", _lib.ptr(u), _lib.ptr(p), _lib.ptr(q1.k), float(limit), *q1._dims(),
+                  _lib.ptr(out), w, n, _lib.stream_handle(q1.device))
+        root, invalid, bad = out.tolist()
+        if bad:
+            raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
+        return float(root), int(invalid), 27 * q1.ncell

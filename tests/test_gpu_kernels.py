@@ -138,6 +138,12 @@ def test_jd_root(gpu_device):
             assert np.isinf(got[0])
         else:
             assert abs(got[0] - want[0]) <= 1e-9 * want[0], (limit, got, want)
+    # the direction as a free-node vector: bitwise the root of its node field
+    pf = Q.restrict_free(v, og)
+    vn = Q.inject_free(pf, og)
+    for limit in (.5, 2., 10.):
+        assert root.from_free(dev(base, gpu_device), dev(pf, gpu_device), limit) == \
+            root(dev(base, gpu_device), dev(vn, gpu_device), limit), limit
 
 
 def test_prescreen_bitwise(gpu_device):

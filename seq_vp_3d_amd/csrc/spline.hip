@@ -17,13 +17,22 @@ namespace seqvp {
 __device__ __forceinline__ int64_t cubic_basis(const double* __restrict__ t, int64_t nt, double x, double b[4],
                                                double db[4]) {
   const int64_t n = nt - 4;
-  // searchsorted(t, x, right=True) - 1: last index i with t[i] <= x
-  int64_t lo = 0, hi = nt;   // invariant: t[j] <= x for j < lo; t[j] > x for j >= hi
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (t[mid] <= x) lo = mid + 1; else hi = mid;
+  // searchsorted(t, x, right=True) - 1: the last index i with t[i] <= x (-1 when there is none, NaN included).
+  // The knots are uniform away from the repeated ends, so a proportional guess lands on or next to the span; the
+  // two walks then end with t[span] <= x < t[span + 1], which for sorted knots is exactly that index (the same
+  // result as a bisection, with one or two independent loads instead of a chain of eight dependent ones).
+  const double t0 = t[0], tl = t[nt - 1];
+  int64_t span;
+  if (!(x >= t0)) {
+    span = -1;
+  } else if (x >= tl) {
+    span = nt - 1;
+  } else {
+    const double f = (x - t0) / (tl - t0) * (double)(nt - 1);
+    span = f < 0.0 ? 0 : (f > (double)(nt - 2) ? nt - 2 : (int64_t)f);
+    while (t[span + 1] <= x) ++span;   // ends: t[nt - 1] > x
+    while (t[span] > x) --span;        // ends: t[0] <= x
   }
-  int64_t span = lo - 1;
   span = span < 3 ? 3 : (span > n - 1 ? n - 1 : span);
   double basis[4] = {1.0, 0.0, 0.0, 0.0};
   double quad[3] = {0.0, 0.0, 0.0};

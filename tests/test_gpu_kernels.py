@@ -202,6 +202,15 @@ def test_spline_warp_and_points(gpu_device):
     v, g = sp.sample(dev(pts, gpu_device))
     wv, wg = osp.sample(ref, pts)
     assert rel(v.cpu().numpy(), wv) <= 1e-12 and rel(g.cpu().numpy(), wg) <= 1e-12
+    # points exactly on nodes (knots), on the support ends and a hair inside them: the span search's edge cases
+    ax = og.axes
+    rng = np.random.default_rng(92)
+    cand = [np.concatenate([a, [a[0], a[-1], np.nextafter(a[0], np.inf), np.nextafter(a[-1], -np.inf)]])
+            for a in ax]
+    pts = np.stack([rng.choice(c, 600) for c in cand], axis=-1)
+    v, g = sp.sample(dev(np.ascontiguousarray(pts), gpu_device))
+    wv, wg = osp.sample(ref, pts)
+    assert rel(v.cpu().numpy(), wv) <= 1e-12 and rel(g.cpu().numpy(), wg) <= 1e-12
 
 
 @pytest.mark.parametrize("fp32", [0, 1])

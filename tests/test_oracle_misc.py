@@ -152,3 +152,34 @@ def test_spline_reproduces_cubics_and_interpolates():
     s2 = spline.build(rough, g)
     v2, _ = spline.sample(s2, X, with_gradient=False)
     np.testing.assert_allclose(v2, rough, atol=1e-12)
+
+
+def test_span_guess_and_walk_is_searchsorted():
+    """The device sampler finds a knot span by a proportional guess and two walks instead of a bisection
+    (csrc/spline.hip cubic_basis); on the not-a-knot knot vectors of uniform grids that procedure returns
+    searchsorted(t, x, right=True) - 1 for every x (knots, ends, between, outside, NaN)."""
+    from scipy.interpolate import make_interp_spline
+
+    def walk(t, x):
+        nt = len(t)
+        if not (x >= t[0]):
+            return -1
+        if x >= t[-1]:
+            return nt - 1
+        f = (x - t[0]) / (t[-1] - t[0]) * (nt - 1)
+        s = 0 if f < 0 else (nt - 2 if f > nt - 2 else int(f))
+        while t[s + 1] <= x:
+            s += 1
+        while t[s] > x:
+            s -= 1
+        return s
+
+    rng = np.random.default_rng(3)
+    for n, h, o in ((8, 1., 0.), (23, .7, -3.), (160, 1., 0.), (224, 1.1, 2.5)):
+        xs = o + h * np.arange(n)
+        t = make_interp_spline(xs, np.zeros(n), k=3).t
+        probes = np.concatenate([t, np.nextafter(t, np.inf), np.nextafter(t, -np.inf), rng.uniform(t[0] - 1, t[-1] + 1, 500),
+                                 [np.nan, -np.inf, np.inf]])
+        for x in probes:
+            want = np.searchsorted(t, x, side="right") - 1 if not np.isnan(x) else -1
+            assert walk(t, x) == want, (n, x)

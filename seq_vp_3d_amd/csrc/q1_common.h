@@ -189,6 +189,28 @@ __device__ __forceinline__ void point_grad_w(const EdgesT<T>& E, const T (&w)[3]
       V[r][j] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], w[j][0], w[j][1], w[j][2], w[j][3]);
 }
 
+// Column j of point_grad_w (V[r][j], r = 0..2) from the weights wj = TABLE.w[p][j]. A column depends only on the
+// point's two other coordinates, so on a 3x3x3 rule the x column (j = 0) is the same for the three points of an
+// x row (p = 3 row + ix): sweeps that walk the points row by row blend it once per row, the same bits.
+template <typename T>
+__device__ __forceinline__ void column_w(const EdgesT<T>& E, int j, const T (&wj)[4], T col[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    col[r] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], wj[0], wj[1], wj[2], wj[3]);
+}
+
+// point_grad_w with the x column given (the row's column_w(E, 0, ...)).
+template <typename T>
+__device__ __forceinline__ void point_grad_yz_w(const EdgesT<T>& E, const T (&w)[3][4], const T x[3], T V[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) V[r][0] = x[r];
+#pragma unroll
+  for (int j = 1; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      V[r][j] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], w[j][0], w[j][1], w[j][2], w[j][3]);
+}
+
 typedef __attribute__((address_space(3))) double lds_double;
 
 // Edges of one field kept in LDS as a per-thread column (element (j, r, k) at col[((j*3 + r)*4 + k)*stride]):
@@ -209,6 +231,29 @@ __device__ __forceinline__ void point_grad_col_w(const volatile lds_double* col,
                                                  double V[3][3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const volatile lds_double* e = col + (j * 3 + r) * 4 * stride;
+      V[r][j] = blend4<double>(e[0], e[stride], e[2 * stride], e[3 * stride], w[j][0], w[j][1], w[j][2], w[j][3]);
+    }
+}
+
+// column_w and point_grad_yz_w for edges kept in LDS.
+__device__ __forceinline__ void column_col_w(const volatile lds_double* col, int stride, int j, const double (&wj)[4],
+                                             double out[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const volatile lds_double* e = col + (j * 3 + r) * 4 * stride;
+    out[r] = blend4<double>(e[0], e[stride], e[2 * stride], e[3 * stride], wj[0], wj[1], wj[2], wj[3]);
+  }
+}
+
+__device__ __forceinline__ void point_grad_col_yz_w(const volatile lds_double* col, int stride,
+                                                    const double (&w)[3][4], const double x[3], double V[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) V[r][0] = x[r];
+#pragma unroll
+  for (int j = 1; j < 3; ++j)
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
       const volatile lds_double* e = col + (j * 3 + r) * 4 * stride;

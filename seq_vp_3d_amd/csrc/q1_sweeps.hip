@@ -150,35 +150,42 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
       }
     }
     double cj = 0.0, cc = 0.0;
+    // points p = 3 row + ix in order; the three points of an x row share each field's x column
 #pragma unroll 1
-    for (int p = 0; p < 27; ++p) {
-      double sp[3];
-      gauss_point(p, sp);
-      const double qw = k[K_QW + p];
-      double V[3][3], W[3][3], C[3], Cr[3];
-      point_grad_w(Eu, GAUSS_W.w[p], V);
-      const double J = jacobian(V);
-      curl_of(V, C);
-      point_grad_col_w(er, BLOCK, GAUSS_W.w[p], W);
-      const double Jr = jacobian(W);
-      curl_of(W, Cr);
-      const double dj = J - Jr;
-      const double d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
-      cj += qw * dj * dj;
-      cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
-      if (WITH_BASE) {
-        // 1/2 w (y - x)(y + x) with y, x the trial and base residuals against the reference: the same stable
-        // difference as with sqrt(w) folded into the residuals; the point's four terms are summed first and
-        // added to the double-double accumulator once.
-        double Y[3][3], Ca[3];
-        point_grad_w(Eb, GAUSS_W.w[p], Y);
-        const double Ja = jacobian(Y);
-        curl_of(Y, Ca);
-        const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
-        double t = wm * ((J - Ja) * (dj + (Ja - Jr)));
+    for (int row = 0; row < 9; ++row) {
+      const double(&wr)[4] = GAUSS_W.w[3 * row][0];
+      double Ux[3], Rx[3], Bx[3];
+      column_w(Eu, 0, wr, Ux);
+      column_col_w(er, BLOCK, 0, wr, Rx);
+      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+#pragma unroll 1
+      for (int p = 3 * row; p < 3 * row + 3; ++p) {
+        const double qw = k[K_QW + p];
+        double V[3][3], W[3][3], C[3], Cr[3];
+        point_grad_yz_w(Eu, GAUSS_W.w[p], Ux, V);
+        const double J = jacobian(V);
+        curl_of(V, C);
+        point_grad_col_yz_w(er, BLOCK, GAUSS_W.w[p], Rx, W);
+        const double Jr = jacobian(W);
+        curl_of(W, Cr);
+        const double dj = J - Jr;
+        const double d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
+        cj += qw * dj * dj;
+        cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
+        if (WITH_BASE) {
+          // 1/2 w (y - x)(y + x) with y, x the trial and base residuals against the reference: the same stable
+          // difference as with sqrt(w) folded into the residuals; the point's four terms are summed first and
+          // added to the double-double accumulator once.
+          double Y[3][3], Ca[3];
+          point_grad_yz_w(Eb, GAUSS_W.w[p], Bx, Y);
+          const double Ja = jacobian(Y);
+          curl_of(Y, Ca);
+          const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
+          double t = wm * ((J - Ja) * (dj + (Ja - Jr)));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) t += wn * ((C[i] - Ca[i]) * ((C[i] - Cr[i]) + (Ca[i] - Cr[i])));
-        dd_add(s[2], t);
+          for (int i = 0; i < 3; ++i) t += wn * ((C[i] - Ca[i]) * ((C[i] - Cr[i]) + (Ca[i] - Cr[i])));
+          dd_add(s[2], t);
+        }
       }
     }
     dd_add(s[0], 0.5 * mu * cj);
@@ -214,21 +221,27 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
       }
     }
     double val = 0.0;
+    // points p = 3 row + ix in order; the three points of an x row share each field's x column
 #pragma unroll 1
-    for (int p = 0; p < 27; ++p) {
-      double sp[3];
-      closed_point(p, sp);
-      double V[3][3];
-      point_grad_w(Eu, CLOSED_W.w[p], V);
-      const double J = jacobian(V);
-      m[0] = nan_min(m[0], J);
-      val += barrier_value(J, w, delta, act, scale);
-      if (WITH_BASE) {
-        double Y[3][3];
-        point_grad_w(Eb, CLOSED_W.w[p], Y);
-        const double Ja = jacobian(Y);
-        m[1] = nan_min(m[1], Ja);
-        if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+    for (int row = 0; row < 9; ++row) {
+      const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+      double Ux[3], Bx[3];
+      column_w(Eu, 0, wr, Ux);
+      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+#pragma unroll 1
+      for (int p = 3 * row; p < 3 * row + 3; ++p) {
+        double V[3][3];
+        point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
+        const double J = jacobian(V);
+        m[0] = nan_min(m[0], J);
+        val += barrier_value(J, w, delta, act, scale);
+        if (WITH_BASE) {
+          double Y[3][3];
+          point_grad_yz_w(Eb, CLOSED_W.w[p], Bx, Y);
+          const double Ja = jacobian(Y);
+          m[1] = nan_min(m[1], Ja);
+          if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+        }
       }
     }
     dd_add(s[0], val);
@@ -385,6 +398,33 @@ __device__ __forceinline__ void adj_accumulate_w(EdgeAdj& G, const double (&w)[3
     }
 }
 
+// adj_accumulate_w for the y and z columns of X; the x column is summed over the points of an x row, which share
+// its weights, and added once per row by adj_accumulate_x.
+__device__ __forceinline__ void adj_accumulate_yz_w(EdgeAdj& G, const double (&w)[3][4], const double X[3][3],
+                                                    double Xx[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) Xx[r] += X[r][0];
+#pragma unroll
+  for (int j = 1; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      G.g[j][r][0] += w[j][0] * X[r][j];
+      G.g[j][r][1] += w[j][1] * X[r][j];
+      G.g[j][r][2] += w[j][2] * X[r][j];
+      G.g[j][r][3] += w[j][3] * X[r][j];
+    }
+}
+
+__device__ __forceinline__ void adj_accumulate_x(EdgeAdj& G, const double (&wr)[4], const double Xx[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    G.g[0][r][0] += wr[0] * Xx[r];
+    G.g[0][r][1] += wr[1] * Xx[r];
+    G.g[0][r][2] += wr[2] * Xx[r];
+    G.g[0][r][3] += wr[3] * Xx[r];
+  }
+}
+
 __device__ __forceinline__ void adj_accumulate(EdgeAdj& G, const double s[3], const double X[3][3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -425,55 +465,70 @@ __device__ __forceinline__ void adj_store(const EdgeAdj& G, const double ih[3], 
 __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, int stride, const double* __restrict__ k,
                                          EdgeAdj& G) {
   const double mu = k[K_MU], nu = k[K_NU];
+  // points p = 3 row + ix in order; the three points of an x row share the x columns (and their weights)
 #pragma unroll 1
-  for (int p = 0; p < 27; ++p) {
-    double sp[3];
-    gauss_point(p, sp);
-    const double qw = k[K_QW + p];
-    double V[3][3], A[3][3], W[3][3], C[3], Cr[3];
-    point_grad_w(Eu, GAUSS_W.w[p], V);
-    const double J = cofactor_jacobian(V, A);
-    curl_of(V, C);
-    point_grad_col_w(er, stride, GAUSS_W.w[p], W);
-    const double Jr = jacobian(W);
-    curl_of(W, Cr);
-    const double jd = mu * qw * (J - Jr);
-    const double c0 = nu * qw * (C[0] - Cr[0]), c1 = nu * qw * (C[1] - Cr[1]), c2 = nu * qw * (C[2] - Cr[2]);
-    double X[3][3];
+  for (int row = 0; row < 9; ++row) {
+    const double(&wr)[4] = GAUSS_W.w[3 * row][0];
+    double Ux[3], Rx[3], Xx[3] = {0.0, 0.0, 0.0};
+    column_w(Eu, 0, wr, Ux);
+    column_col_w(er, stride, 0, wr, Rx);
+#pragma unroll 1
+    for (int p = 3 * row; p < 3 * row + 3; ++p) {
+      const double qw = k[K_QW + p];
+      double V[3][3], A[3][3], W[3][3], C[3], Cr[3];
+      point_grad_yz_w(Eu, GAUSS_W.w[p], Ux, V);
+      const double J = cofactor_jacobian(V, A);
+      curl_of(V, C);
+      point_grad_col_yz_w(er, stride, GAUSS_W.w[p], Rx, W);
+      const double Jr = jacobian(W);
+      curl_of(W, Cr);
+      const double jd = mu * qw * (J - Jr);
+      const double c0 = nu * qw * (C[0] - Cr[0]), c1 = nu * qw * (C[1] - Cr[1]), c2 = nu * qw * (C[2] - Cr[2]);
+      double X[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * jd;
-    X[2][1] += c0;
-    X[1][2] -= c0;
-    X[0][2] += c1;
-    X[2][0] -= c1;
-    X[1][0] += c2;
-    X[0][1] -= c2;
-    adj_accumulate_w(G, GAUSS_W.w[p], X);
+        for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * jd;
+      X[2][1] += c0;
+      X[1][2] -= c0;
+      X[0][2] += c1;
+      X[2][0] -= c1;
+      X[1][0] += c2;
+      X[0][1] -= c2;
+      adj_accumulate_yz_w(G, GAUSS_W.w[p], X, Xx);
+    }
+    adj_accumulate_x(G, wr, Xx);
   }
 }
 
 __device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __restrict__ k, EdgeAdj& G) {
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+  // points p = 3 row + ix in order; the three points of an x row share the x columns (and their weights)
 #pragma unroll 1
-  for (int p = 0; p < 27; ++p) {
-    double sp[3];
-    closed_point(p, sp);
-    double V[3][3];
-    point_grad_w(Eu, CLOSED_W.w[p], V);
-    // psi'(J) is exactly 0 at and above the activation: a point where every lane of the wave is
-    // inactive contributes nothing and is skipped (wave-uniform branch).
-    if (!__any(jacobian(V) < act)) continue;
-    double A[3][3];
-    const double J = cofactor_jacobian(V, A);
-    const double b1 = barrier_first(J, w, delta, act, scale);
-    double X[3][3];
+  for (int row = 0; row < 9; ++row) {
+    const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+    double Ux[3], Xx[3] = {0.0, 0.0, 0.0};
+    column_w(Eu, 0, wr, Ux);
+    bool any = false;
+#pragma unroll 1
+    for (int p = 3 * row; p < 3 * row + 3; ++p) {
+      double V[3][3];
+      point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
+      // psi'(J) is exactly 0 at and above the activation: a point where every lane of the wave is
+      // inactive contributes nothing and is skipped (wave-uniform branch).
+      if (!__any(jacobian(V) < act)) continue;
+      any = true;
+      double A[3][3];
+      const double J = cofactor_jacobian(V, A);
+      const double b1 = barrier_first(J, w, delta, act, scale);
+      double X[3][3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * b1;
-    adj_accumulate_w(G, CLOSED_W.w[p], X);
+        for (int j = 0; j < 3; ++j) X[r][j] = A[r][j] * b1;
+      adj_accumulate_yz_w(G, CLOSED_W.w[p], X, Xx);
+    }
+    if (any) adj_accumulate_x(G, wr, Xx);
   }
 }
 
@@ -958,15 +1013,21 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
         load_cell(v, b, d, w);
       make_edges(w, ih, Ev);
     }
-    for (int p = 0; p < 27; ++p) {
-      double sp[3];
-      closed_point(p, sp);
+    // points p = 3 row + ix in order; the three points of an x row share each field's x column
+#pragma unroll 1
+    for (int row = 0; row < 9; ++row) {
+    const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+    double Ux[3], Vx[3];
+    column_w(Eu, 0, wr, Ux);
+    column_w(Ev, 0, wr, Vx);
+#pragma unroll 1
+    for (int p = 3 * row; p < 3 * row + 3; ++p) {
       double F[3][3], V[3][3], AF[3][3], AV[3][3];
-      point_grad_w(Eu, CLOSED_W.w[p], F);
+      point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, F);
       F[0][0] += 1.0;
       F[1][1] += 1.0;
       F[2][2] += 1.0;
-      point_grad_w(Ev, CLOSED_W.w[p], V);
+      point_grad_yz_w(Ev, CLOSED_W.w[p], Vx, V);
       cofactor_plain(F, AF);
       cofactor_plain(V, AV);
       const double c0 = ((F[0][0] * AF[0][0] + F[0][1] * AF[0][1]) + F[0][2] * AF[0][2]) - delta;
@@ -990,6 +1051,7 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
       }
       if (c0 > fabs(b1) + fabs(b2) + fabs(b3)) continue;
       best = fmin(best, first_root(c0, b1, b2, b3));
+    }
     }
   }
   // min over the block; counts summed (exact integers in double)

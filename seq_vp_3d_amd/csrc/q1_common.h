@@ -12,6 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace seqvp {
 
 // Constant vector layout shared with the host (39 doubles; triton_q1_kernels.py:1-6 layout).
@@ -209,6 +212,73 @@ __device__ __forceinline__ void point_grad_yz_w(const EdgesT<T>& E, const T (&w)
 #pragma unroll
     for (int r = 0; r < 3; ++r)
       V[r][j] = blend4(E.e[j][r][0], E.e[j][r][1], E.e[j][r][2], E.e[j][r][3], w[j][0], w[j][1], w[j][2], w[j][3]);
+}
+
+// ---- the closed rule with compile-time points ------------------------------------------------------------------
+// On the closed 3x3x3 rule (s in {0, 1/2, 1}) every blend weight is 0, 1/4, 1/2 or 1. With the point known at
+// compile time a term of weight 0 is dropped (it adds only a signed zero to the chain) and a weight of 1 is not
+// multiplied (exact), so a blend costs 0 (corner), 2 (edge midpoint) or 4 (face centre) instructions instead of
+// 4 and gives the blend4 value up to the sign of an exact zero. Points whose blends coincide compute the same
+// expression, which the compiler computes once.
+// closed_f2(k, S) = 2 f(k, s) for s = S/2: the per-axis factor (1 - s or s) times 2; a weight times 4 is a product
+// of two of them.
+__host__ __device__ constexpr int closed_f2(int k, int S) { return k ? S : 2 - S; }
+
+template <int Q4, bool FIRST, typename T>
+__device__ __forceinline__ T closed_term(T acc, T e) {
+  if constexpr (Q4 == 0) {
+    return acc;
+  } else if constexpr (FIRST) {
+    if constexpr (Q4 == 4) return e;
+    else return e * (T)(Q4 * 0.25);
+  } else {
+    return fma(e, (T)(Q4 * 0.25), acc);
+  }
+}
+
+// blend4 of the four j-edges at a closed point whose two other-axis coordinates are S1/2 and S2/2.
+template <int S1, int S2, typename T>
+__device__ __forceinline__ T closed_blend(T e0, T e1, T e2, T e3) {
+  constexpr int q00 = closed_f2(0, S1) * closed_f2(0, S2), q10 = closed_f2(1, S1) * closed_f2(0, S2);
+  constexpr int q01 = closed_f2(0, S1) * closed_f2(1, S2), q11 = closed_f2(1, S1) * closed_f2(1, S2);
+  T a = closed_term<q00, true>((T)0, e0);
+  a = closed_term<q10, q00 == 0>(a, e1);
+  a = closed_term<q01, q00 == 0 && q10 == 0>(a, e2);
+  return closed_term<q11, q00 == 0 && q10 == 0 && q01 == 0>(a, e3);
+}
+
+// point_grad_w at closed point P = 9 iz + 3 iy + ix.
+template <int P, typename T>
+__device__ __forceinline__ void closed_point_grad(const EdgesT<T>& E, T V[3][3]) {
+  constexpr int S[3] = {P % 3, (P / 3) % 3, P / 9};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    V[r][0] = closed_blend<S[1], S[2]>(E.e[0][r][0], E.e[0][r][1], E.e[0][r][2], E.e[0][r][3]);
+    V[r][1] = closed_blend<S[0], S[2]>(E.e[1][r][0], E.e[1][r][1], E.e[1][r][2], E.e[1][r][3]);
+    V[r][2] = closed_blend<S[0], S[1]>(E.e[2][r][0], E.e[2][r][1], E.e[2][r][2], E.e[2][r][3]);
+  }
+}
+
+// Hides the edges' values from the compiler (no instruction): blends after it are not merged with the same blends
+// before it, which bounds how long shared blends stay live (register pressure) when points are unrolled.
+template <typename T>
+__device__ __forceinline__ void opaque_edges(EdgesT<T>& E) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(E.e[j][r][k]));
+}
+
+// f(std::integral_constant<int, I>) for I = 0 .. N-1 in order (a loop whose index is a compile-time constant).
+template <typename Fn, int... I>
+__device__ __forceinline__ void static_for_seq(Fn&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
 typedef __attribute__((address_space(3))) double lds_double;

@@ -220,31 +220,54 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
         make_edges(v, ih, Eb);
       }
     }
-    double val = 0.0;
-    // points p = 3 row + ix in order; the three points of an x row share each field's x column
+    // Minima over the 27 points, each with its compile-time closed blend, and whether any point is active.
+    bool active = false;
+    static_for<27>([&](auto P) {
+      if constexpr (P % 9 == 0) {
+        opaque_edges(Eu);
+        if (WITH_BASE) opaque_edges(Eb);
+      }
+      double V[3][3];
+      closed_point_grad<P>(Eu, V);
+      const double J = jacobian(V);
+      m[0] = nan_min(m[0], J);
+      if (WITH_BASE) {
+        double Y[3][3];
+        closed_point_grad<P>(Eb, Y);
+        const double Ja = jacobian(Y);
+        m[1] = nan_min(m[1], Ja);
+        active |= !(J >= act && Ja >= act);
+      } else {
+        active |= J < act;
+      }
+    });
+    // The barrier terms are zero at every point with J >= act (and, against a base, Ja >= act), so only a wave
+    // with an active point evaluates them, over the points in order with the tabulated blends.
+    if (__any(active)) {
+      double val = 0.0;
+      // points p = 3 row + ix in order; the three points of an x row share each field's x column
 #pragma unroll 1
-    for (int row = 0; row < 9; ++row) {
-      const double(&wr)[4] = CLOSED_W.w[3 * row][0];
-      double Ux[3], Bx[3];
-      column_w(Eu, 0, wr, Ux);
-      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+      for (int row = 0; row < 9; ++row) {
+        const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+        double Ux[3], Bx[3];
+        column_w(Eu, 0, wr, Ux);
+        if (WITH_BASE) column_w(Eb, 0, wr, Bx);
 #pragma unroll 1
-      for (int p = 3 * row; p < 3 * row + 3; ++p) {
-        double V[3][3];
-        point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
-        const double J = jacobian(V);
-        m[0] = nan_min(m[0], J);
-        val += barrier_value(J, w, delta, act, scale);
-        if (WITH_BASE) {
-          double Y[3][3];
-          point_grad_yz_w(Eb, CLOSED_W.w[p], Bx, Y);
-          const double Ja = jacobian(Y);
-          m[1] = nan_min(m[1], Ja);
-          if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+        for (int p = 3 * row; p < 3 * row + 3; ++p) {
+          double V[3][3];
+          point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
+          const double J = jacobian(V);
+          val += barrier_value(J, w, delta, act, scale);
+          if (WITH_BASE) {
+            double Y[3][3];
+            point_grad_yz_w(Eb, CLOSED_W.w[p], Bx, Y);
+            const double Ja = jacobian(Y);
+            if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+          }
         }
       }
+      dd_add(s[0], val);
     }
-    dd_add(s[0], val);
   }
   block_reduce_store<2, 2>(s, m, partials + blk * 6);
 }
@@ -501,8 +524,18 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, 
   }
 }
 
-__device__ __forceinline__ void barrier_vjp(const Edges& Eu, const double* __restrict__ k, EdgeAdj& G) {
+__device__ __forceinline__ void barrier_vjp(Edges& Eu, const double* __restrict__ k, EdgeAdj& G) {
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+  // psi'(J) is exactly 0 at and above the activation: J at the 27 points with their compile-time closed blends
+  // first, and a wave with no point below it contributes nothing.
+  bool active = false;
+  static_for<27>([&](auto P) {
+    if constexpr (P % 9 == 0) opaque_edges(Eu);
+    double V[3][3];
+    closed_point_grad<P>(Eu, V);
+    active |= jacobian(V) < act;
+  });
+  if (!__any(active)) return;
   // points p = 3 row + ix in order; the three points of an x row share the x columns (and their weights)
 #pragma unroll 1
   for (int row = 0; row < 9; ++row) {
@@ -623,8 +656,17 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
       }
     }
   }
+  // psi''(J) is exactly 0 at and above the activation: J at the 27 points with their compile-time closed blends
+  // first, and a wave with no point below it skips the barrier part.
+  bool active = false;
+  static_for<27>([&](auto P) {
+    if constexpr (P % 9 == 0) opaque_edges(Eu);
+    double V[3][3];
+    closed_point_grad<P>(Eu, V);
+    active |= jacobian(V) < act;
+  });
 #pragma unroll 1
-  for (int p = 0; p < 27; ++p) {
+  for (int p = 0; p < 27 && __any(active); ++p) {
     double sp[3];
     closed_point(p, sp);
     double V[3][3], A[3][3];
@@ -704,13 +746,11 @@ __device__ __forceinline__ void cell_closed_jd(const double* __restrict__ u, con
     load_cell(u, cell_base_node(c, d), d, v);
     make_edges(v, ih, Eu);
   }
-#pragma unroll
-  for (int p = 0; p < 27; ++p) {
-    double sp[3], V[3][3];
-    closed_point(p, sp);
-    point_grad_w(Eu, CLOSED_W.w[p], V);
-    J[p] = jacobian(V);
-  }
+  static_for<27>([&](auto P) {
+    double V[3][3];
+    closed_point_grad<P>(Eu, V);
+    J[P] = jacobian(V);
+  });
 }
 
 // m_c per cell. With EXT, a cell whose Bernstein hull reaches below min(m_c, activation) is appended to the
@@ -986,6 +1026,28 @@ __device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3]
   }
 }
 
+// The T66 cubic det(F + s V) - delta = c0 + c1 s + c2 s^2 + c3 s^3 at one point (F = I + grad u, V = grad v),
+// scaled to s/L: b_i = c_i L^i (global_jd_step.py:90-129).
+__device__ __forceinline__ void root_cubic(const double F[3][3], const double V[3][3], double delta, double L,
+                                           double L2, double L3, double& c0, double& b1, double& b2, double& b3) {
+  double AF[3][3], AV[3][3];
+  cofactor_plain(F, AF);
+  cofactor_plain(V, AV);
+  c0 = ((F[0][0] * AF[0][0] + F[0][1] * AF[0][1]) + F[0][2] * AF[0][2]) - delta;
+  double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      c1 += AF[r][j] * V[r][j];
+      c2 += F[r][j] * AV[r][j];
+    }
+  const double c3 = (V[0][0] * AV[0][0] + V[0][1] * AV[0][1]) + V[0][2] * AV[0][2];
+  b1 = c1 * L;
+  b2 = c2 * L2;
+  b3 = c3 * L3;
+}
+
 // Per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129; torch_ops.py:636-673).
 // FREE: v is the direction's free-node vector (the node values of inject(v), read without the node field).
 template <bool FREE>
@@ -1013,45 +1075,54 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
         load_cell(v, b, d, w);
       make_edges(w, ih, Ev);
     }
-    // points p = 3 row + ix in order; the three points of an x row share each field's x column
-#pragma unroll 1
-    for (int row = 0; row < 9; ++row) {
-    const double(&wr)[4] = CLOSED_W.w[3 * row][0];
-    double Ux[3], Vx[3];
-    column_w(Eu, 0, wr, Ux);
-    column_w(Ev, 0, wr, Vx);
-#pragma unroll 1
-    for (int p = 3 * row; p < 3 * row + 3; ++p) {
-      double F[3][3], V[3][3], AF[3][3], AV[3][3];
-      point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, F);
+    // Pass 1: every point with its compile-time closed blend; nonfinite and c0 <= 0 points are counted, and a
+    // point whose cubic may have a root in (0, 1] marks the lane.
+    bool need = false;
+    static_for<27>([&](auto P) {
+      if constexpr (P % 9 == 0) {
+        opaque_edges(Eu);
+        opaque_edges(Ev);
+      }
+      double F[3][3], V[3][3];
+      closed_point_grad<P>(Eu, F);
       F[0][0] += 1.0;
       F[1][1] += 1.0;
       F[2][2] += 1.0;
-      point_grad_yz_w(Ev, CLOSED_W.w[p], Vx, V);
-      cofactor_plain(F, AF);
-      cofactor_plain(V, AV);
-      const double c0 = ((F[0][0] * AF[0][0] + F[0][1] * AF[0][1]) + F[0][2] * AF[0][2]) - delta;
-      double c1 = 0.0, c2 = 0.0;
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          c1 += AF[r][j] * V[r][j];
-          c2 += F[r][j] * AV[r][j];
-        }
-      const double c3 = (V[0][0] * AV[0][0] + V[0][1] * AV[0][1]) + V[0][2] * AV[0][2];
-      const double b1 = c1 * L, b2 = c2 * L2, b3 = c3 * L3;
-      if (!(isfinite(c0) && isfinite(b1) && isfinite(b2) && isfinite(b3))) {
+      closed_point_grad<P>(Ev, V);
+      double c0, b1, b2, b3;
+      root_cubic(F, V, delta, L, L2, L3, c0, b1, b2, b3);
+      if (!(isfinite(c0) && isfinite(b1) && isfinite(b2) && isfinite(b3)))
         bad = 1.0;
-        continue;
-      }
-      if (!(c0 > 0.0)) {
+      else if (!(c0 > 0.0))
         invalid += 1.0;
-        continue;
+      else
+        need |= !(c0 > fabs(b1) + fabs(b2) + fabs(b3));
+    });
+    // Pass 2, for a wave with a marked lane: the roots, over the points in order with the tabulated blends (the
+    // same coefficients up to the sign of an exact zero, so the same points search).
+    if (__any(need)) {
+      // points p = 3 row + ix in order; the three points of an x row share each field's x column
+#pragma unroll 1
+      for (int row = 0; row < 9; ++row) {
+        const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+        double Ux[3], Vx[3];
+        column_w(Eu, 0, wr, Ux);
+        column_w(Ev, 0, wr, Vx);
+#pragma unroll 1
+        for (int p = 3 * row; p < 3 * row + 3; ++p) {
+          double F[3][3], V[3][3];
+          point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, F);
+          F[0][0] += 1.0;
+          F[1][1] += 1.0;
+          F[2][2] += 1.0;
+          point_grad_yz_w(Ev, CLOSED_W.w[p], Vx, V);
+          double c0, b1, b2, b3;
+          root_cubic(F, V, delta, L, L2, L3, c0, b1, b2, b3);
+          if (isfinite(c0) && isfinite(b1) && isfinite(b2) && isfinite(b3) && c0 > 0.0 &&
+              !(c0 > fabs(b1) + fabs(b2) + fabs(b3)))
+            best = fmin(best, first_root(c0, b1, b2, b3));
+        }
       }
-      if (c0 > fabs(b1) + fabs(b2) + fabs(b3)) continue;
-      best = fmin(best, first_root(c0, b1, b2, b3));
-    }
     }
   }
   // min over the block; counts summed (exact integers in double)

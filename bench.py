@@ -32,15 +32,16 @@ FLOPS_PER_UNIT = {
 }
 # FP64 operations the build's kernels actually execute per cell (per node for the sampler): SQ_INSTS_VALU_FLOPS_FP64
 # x 64 lanes per dispatch / units (rocprofv3 --pmc, profiles/valu_issue_native_pmc.md). The edge-difference
-# geometry and the tabulated blend weights do the reference's work with about half of its operation count.
+# geometry, the tabulated blend weights and the compile-time closed points do the reference's work with a third to a
+# half of its operation count.
 EXECUTED_FLOPS_PER_UNIT = {
-    "seqvp_q1_soft_energy": 8411.,
-    "seqvp_q1_barrier_energy": 4603.,
-    "seqvp_q1_gradient": 10197. + 24.,
-    "seqvp_q1_gradient_free": 10197. + 24.,
-    "seqvp_q1_ggn_diagonal": 14167.,
-    "seqvp_jd_root": 6536.,
-    "seqvp_spline_warp": 967.,
+    "seqvp_q1_soft_energy": 7223.,
+    "seqvp_q1_barrier_energy": 1620.,
+    "seqvp_q1_gradient": 7542. + 24.,
+    "seqvp_q1_gradient_free": 7542. + 24.,
+    "seqvp_q1_ggn_diagonal": 12529.,
+    "seqvp_jd_root": 3646.,
+    "seqvp_spline_warp": 1015.,
 }
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch at 160x192x224 from the FETCH_SIZE and WRITE_SIZE counter runs (rocprofv3 --pmc, one counter

@@ -304,32 +304,41 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_f32_kernel(const double* __
       }
     }
     KahanF cj{0.f, 0.f}, cc{0.f, 0.f}, cd{0.f, 0.f};
+    // points p = 3 row + ix in order; the three points of an x row share each field's x column
 #pragma unroll 1
-    for (int p = 0; p < 27; ++p) {
-      const float qw = (float)k[K_QW + p];
-      float V[3][3], W[3][3], C[3], Cr[3];
-      point_grad_w(Eu, GAUSS_WF.w[p], V);
-      const float J = jacobian(V);
-      curl_of(V, C);
-      point_grad_w(Er, GAUSS_WF.w[p], W);
-      const float Jr = jacobian(W);
-      curl_of(W, Cr);
-      const float dj = J - Jr;
-      const float d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
-      kahan_add(cj, qw * dj * dj);
-      kahan_add(cc, (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2);
-      if (WITH_BASE) {
-        float Y[3][3], Ca[3];
-        point_grad_w(Eb, GAUSS_WF.w[p], Y);
-        const float Ja = jacobian(Y);
-        curl_of(Y, Ca);
-        const float sm = sqrtf(mu * qw), sn = sqrtf(nu * qw);
-        const float xj = sm * (Ja - Jr), yj = sm * dj;
-        kahan_add(cd, 0.5f * ((yj - xj) * (yj + xj)));
+    for (int row = 0; row < 9; ++row) {
+      const float(&wr)[4] = GAUSS_WF.w[3 * row][0];
+      float Ux[3], Rx[3], Bx[3];
+      column_w(Eu, 0, wr, Ux);
+      column_w(Er, 0, wr, Rx);
+      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+#pragma unroll 1
+      for (int p = 3 * row; p < 3 * row + 3; ++p) {
+        const float qw = (float)k[K_QW + p];
+        float V[3][3], W[3][3], C[3], Cr[3];
+        point_grad_yz_w(Eu, GAUSS_WF.w[p], Ux, V);
+        const float J = jacobian(V);
+        curl_of(V, C);
+        point_grad_yz_w(Er, GAUSS_WF.w[p], Rx, W);
+        const float Jr = jacobian(W);
+        curl_of(W, Cr);
+        const float dj = J - Jr;
+        const float d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
+        kahan_add(cj, qw * dj * dj);
+        kahan_add(cc, (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2);
+        if (WITH_BASE) {
+          float Y[3][3], Ca[3];
+          point_grad_yz_w(Eb, GAUSS_WF.w[p], Bx, Y);
+          const float Ja = jacobian(Y);
+          curl_of(Y, Ca);
+          const float sm = sqrtf(mu * qw), sn = sqrtf(nu * qw);
+          const float xj = sm * (Ja - Jr), yj = sm * dj;
+          kahan_add(cd, 0.5f * ((yj - xj) * (yj + xj)));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const float xc = sn * (Ca[i] - Cr[i]), yc = sn * (C[i] - Cr[i]);
-          kahan_add(cd, 0.5f * ((yc - xc) * (yc + xc)));
+          for (int i = 0; i < 3; ++i) {
+            const float xc = sn * (Ca[i] - Cr[i]), yc = sn * (C[i] - Cr[i]);
+            kahan_add(cd, 0.5f * ((yc - xc) * (yc + xc)));
+          }
         }
       }
     }
@@ -365,19 +374,41 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_f32_kernel(const double*
     }
     KahanF val{0.f, 0.f}, chg{0.f, 0.f};
     float m0 = __builtin_inff(), m1 = __builtin_inff();
-#pragma unroll 1
-    for (int p = 0; p < 27; ++p) {
+    // minima over the 27 points with their compile-time closed blends, and whether any point is active
+    bool active = false;
+    static_for<27>([&](auto P) {
+      if constexpr (P % 9 == 0) {
+        opaque_edges(Eu);
+        if (WITH_BASE) opaque_edges(Eb);
+      }
       float V[3][3];
-      point_grad_w(Eu, CLOSED_WF.w[p], V);
+      closed_point_grad<P>(Eu, V);
       const float J = jacobian(V);
       m0 = (J != J || m0 != m0) ? __builtin_nanf("") : (J < m0 ? J : m0);
-      kahan_add(val, barrier_value(J, w, delta, act, scale));
       if (WITH_BASE) {
         float Y[3][3];
-        point_grad_w(Eb, CLOSED_WF.w[p], Y);
+        closed_point_grad<P>(Eb, Y);
         const float Ja = jacobian(Y);
         m1 = (Ja != Ja || m1 != m1) ? __builtin_nanf("") : (Ja < m1 ? Ja : m1);
-        if (!(J >= act && Ja >= act)) kahan_add(chg, barrier_change(Ja, J, delta, act, scale));
+        active |= !(J >= act && Ja >= act);
+      } else {
+        active |= J < act;
+      }
+    });
+    // the barrier sums are (0, 0) unless a point is active: a wave with one runs the full sequence of points
+    if (__any(active)) {
+#pragma unroll 1
+      for (int p = 0; p < 27; ++p) {
+        float V[3][3];
+        point_grad_w(Eu, CLOSED_WF.w[p], V);
+        const float J = jacobian(V);
+        kahan_add(val, barrier_value(J, w, delta, act, scale));
+        if (WITH_BASE) {
+          float Y[3][3];
+          point_grad_w(Eb, CLOSED_WF.w[p], Y);
+          const float Ja = jacobian(Y);
+          if (!(J >= act && Ja >= act)) kahan_add(chg, barrier_change(Ja, J, delta, act, scale));
+        }
       }
     }
     dd_add(s[0], (double)val.s - (double)val.c);

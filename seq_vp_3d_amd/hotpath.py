@@ -123,6 +123,7 @@ class PairState:
         self.cert = DeviceCertificate(g, self.device, config)
         self._certificate = None    # certificate of the current candidate (kept across its energy evaluations)
         self.certificate = None     # certificate of the accepted state (None: not computed yet)
+        self._witness_layout = None   # (key, cells, points) on the device, the last witness layout
         self.poisson = None
         self._den = None
         self.energy = None          # Energies of the accepted state
@@ -243,12 +244,24 @@ class PairState:
     def jd_bound(self, limit):
         return self.root.from_free(self.u, self.p, limit)
 
+    def jd_bound_launch(self, limit):
+        """jd_bound's kernels enqueued; jd_bound_read(pending) returns its result (the host can work meanwhile)."""
+        return self.root.from_free_launch(self.u, self.p, limit)
+
+    def jd_bound_read(self, pending):
+        return self.root.read(pending)
+
     def witness_coefficients(self, cells, points):
         """(n, 4) host array: det(A + s V) - delta at reference points of cells, A = I + grad u of the accepted
         state, V = grad of the direction p (jd_step.py:13-17, 204-222)."""
         self.inject(self.p, self.v)
-        c = torch.as_tensor(np.ascontiguousarray(cells, dtype=np.int64)).to(self.device)
-        pts = torch.as_tensor(np.ascontiguousarray(points, dtype=np.float64)).to(self.device)
+        cells = np.ascontiguousarray(cells, dtype=np.int64)
+        points = np.ascontiguousarray(points, dtype=np.float64)
+        key = (cells.tobytes(), points.tobytes())
+        if self._witness_layout is None or self._witness_layout[0] != key:
+            # the witness layout changes only when a witness is added: uploaded once per layout
+            self._witness_layout = (key, torch.as_tensor(cells).to(self.device), torch.as_tensor(points).to(self.device))
+        _, c, pts = self._witness_layout
         out = torch.empty((len(cells), 4), dtype=torch.float64, device=self.device)
         nz, ny, nx = self.shape
         _lib.call("seqvp_witness_coefficients", _lib.ptr(self.u), _lib.ptr(self.v), _lib.ptr(self.q1.k), nx, ny, nz,

@@ -231,6 +231,10 @@ class FusedJDRoot:
 
     def from_free(self, u, p, limit):
         """The same bound with the direction as its free-node vector p (no node field)."""
+        return self.read(self.from_free_launch(u, p, limit))
+
+    def from_free_launch(self, u, p, limit):
+        """Enqueues from_free's kernels on the device's stream and returns the pending result for read()."""
         q1 = self.q1
         q1._check(u)
         nz, ny, nx = q1.shape
@@ -243,7 +247,11 @@ class FusedJDRoot:
         w, n = q1._workspace(OP_ROOT)
         _lib.call("seqvp_jd_root_free", _lib.ptr(u), _lib.ptr(p), _lib.ptr(q1.k), float(limit), *q1._dims(),
                   _lib.ptr(out), w, n, _lib.stream_handle(q1.device))
+        return out
+
+    def read(self, out):
+        """(root, invalid, points) of a from_free_launch result (waits for it)."""
         root, invalid, bad = out.tolist()
         if bad:
             raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
-        return float(root), int(invalid), 27 * q1.ncell
+        return float(root), int(invalid), 27 * self.q1.ncell

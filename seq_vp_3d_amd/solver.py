@@ -272,12 +272,14 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             return finish("domain_blocked", "NO_POSITIVE_DOMAIN_STEP")
         tau = min(proposal, .9 * bound)
         # JD step prediction: the historical witnesses, then (T66) the global closed27 root on the same limit
-        # (poisson_gradient.py:1356-1366, 708-735)
+        # (poisson_gradient.py:1356-1366, 708-735); the root's sweep is enqueued first and runs while the host
+        # evaluates the witnesses
+        pending_root = st.jd_bound_launch(tau) if config.jd_step_global else None
         predicted, prediction = memory.predict(st.witness_coefficients, tau, config.jd_step_safety)
         root_info = None
         if config.jd_step_global:
             try:
-                root_s, invalid, points = st.jd_bound(tau)
+                root_s, invalid, points = st.jd_bound_read(pending_root)
             except FloatingPointError as exc:
                 root_s, invalid, points = float("nan"), 0, 0
                 root_info = dict(fallback=str(exc))

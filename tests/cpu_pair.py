@@ -96,6 +96,12 @@ class CpuPairState:
         v = Q.inject_free(self.p.numpy(), self.grid)
         return ojr.global_root(self.u, v, self.grid, self.D, self.m.delta, limit)
 
+    def jd_bound_launch(self, limit):
+        return limit
+
+    def jd_bound_read(self, pending):
+        return self.jd_bound(pending)
+
     def witness_coefficients(self, cells, points):
         v = Q.inject_free(self.p.numpy(), self.grid)
         out = []

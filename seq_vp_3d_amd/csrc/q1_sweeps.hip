@@ -4,9 +4,10 @@
 // One cell per lane, 256-lane workgroups (4 wave64), block ids remapped per XCD. FP64 VALU bound:
 // 8-27 kflop per cell against 72-216 B of node reads. Energies reduce in double-double (wave
 // butterfly, then a fixed-order block and grid pass). Node gathers are deterministic: each cell
-// writes its 24 vertex contributions to a structure-of-arrays buffer loc[a*3+r][cell] (coalesced
-// stores), and one lane per node adds the eight contributions in vertex order a = 0..7 after the
-// node base -- no atomics, so every result is bit-reproducible.
+// writes its vertex contributions to a structure-of-arrays buffer loc[a*3+r][cell] (coalesced
+// stores; x-neighbour pairs in one wave are added by a lane shuffle and stored once), and one lane
+// per node adds them in a fixed order after the node base -- no atomics, so every result is
+// bit-reproducible.
 #include "q1_common.h"
 #include "jd_extension.h"
 #include "abi_util.h"
@@ -500,19 +501,70 @@ __device__ __forceinline__ constexpr int vertex_k(int a, int j) {
   return ((a >> other1(j)) & 1) | (((a >> other2(j)) & 1) << 1);
 }
 
-__device__ __forceinline__ void adj_store(const EdgeAdj& G, const double ih[3], double* __restrict__ loc, int64_t c,
-                                          int64_t ncell) {
+// ---- per-cell vertex contributions ("locals") and their gathers -------------------------------------------------
+// Vertex a + 1 (x bit 1) of cell c is vertex a (x bit 0) of cell c + 1 when c + 1 is in the same x row. When c + 1
+// is also in the same wave (lane = c mod 64: the sweeps map cell blk*BLOCK + threadIdx.x) the pair is added there
+// by a lane shuffle and stored once: plane a (x bit 0) of cell c holds v_a(c) + v_{a+1}(c - 1) where c merges, and
+// plane a + 1 holds v_{a+1}(c) only for the cells whose successor does not merge (a wave's last lane, a row's last
+// cell); its other entries are never written or read. Halves the locals traffic of the sweeps and gathers.
+__device__ __forceinline__ bool locals_merge(int64_t c, int64_t x) { return (c & 63) != 0 && x != 0; }
+__device__ __forceinline__ bool locals_keep_odd(int64_t c, int64_t x, const Dims& d) {
+  return ((c + 1) & 63) == 0 || x == d.cx - 1;
+}
+
+// Stores the pair (a, a + 1) of component r; every lane of the wave calls it (the shuffle), `valid` lanes store.
+__device__ __forceinline__ void locals_store_pair(double even, double odd, double* __restrict__ loc, int a, int r,
+                                                  int64_t c, int64_t x, bool valid, const Dims& d) {
+  const double prev = __shfl_up(odd, 1, 64);
+  const double m = locals_merge(c, x) ? even + prev : even;
+  if (valid) {
+    loc[(int64_t)(a * 3 + r) * d.ncell + c] = m;
+    if (locals_keep_odd(c, x, d)) loc[(int64_t)((a + 1) * 3 + r) * d.ncell + c] = odd;
+  }
+}
+
+// Adds the contributions of the cells around node (x, y, z) to acc in a fixed order: for each (y, z) offset of
+// the four pairs, the +x cell's plane (merged with the -x cell where it merges), then the -x cell's own plane
+// where it was not merged. EDGE: the node may lie on the boundary (cells out of range are skipped).
+template <bool EDGE>
+__device__ __forceinline__ double locals_gather(const double* __restrict__ loc, int r, int64_t x, int64_t y, int64_t z,
+                                                const Dims& d, double acc) {
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int pr = 0; pr < 4; ++pr) {
+    const int64_t ya = y - (pr & 1), za = z - (pr >> 1);
+    if (EDGE && (ya < 0 || ya >= d.cy || za < 0 || za >= d.cz)) continue;
+    const int a = 2 * pr;
+    const int64_t ce = (za * d.cy + ya) * d.cx + x;   // the +x cell (x bit 0 vertex); its -x neighbour is ce - 1
+    const bool has_e = !EDGE || x < d.cx, has_o = !EDGE || x > 0;
+    bool merged = false;
+    if (has_e) {
+      acc += loc[(int64_t)(a * 3 + r) * d.ncell + ce];
+      merged = locals_merge(ce, x);
+    }
+    if (has_o && !merged) acc += loc[(int64_t)((a + 1) * 3 + r) * d.ncell + ce - 1];
+  }
+  return acc;
+}
+
+__device__ __forceinline__ void adj_store(const EdgeAdj& G, const double ih[3], double* __restrict__ loc, int64_t c,
+                                          int64_t x, bool valid, const Dims& d) {
+#pragma unroll
+  for (int a = 0; a < 8; a += 2)
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      double v = 0.0;
+      double v[2];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double t = ih[j] * G.g[j][r][vertex_k(a, j)];
-        v = ((a >> j) & 1) ? v + t : v - t;
+      for (int q = 0; q < 2; ++q) {
+        const int b = a + q;
+        double t0 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double t = ih[j] * G.g[j][r][vertex_k(b, j)];
+          t0 = ((b >> j) & 1) ? t0 + t : t0 - t;
+        }
+        v[q] = t0;
       }
-      loc[(int64_t)(a * 3 + r) * ncell + c] = v;
+      locals_store_pair(v[0], v[1], loc, a, r, c, x, valid, d);
     }
 }
 
@@ -602,11 +654,15 @@ template <int MODE, bool SOFT, bool BARRIER>
 __global__ void __launch_bounds__(BLOCK, 2) grad_locals_kernel(const double* __restrict__ u, const double* __restrict__ ref,
                                                             const double* __restrict__ k, Dims d,
                                                             double* __restrict__ loc, double* __restrict__ loc2) {
+  // every lane runs to the end (the locals' lane shuffle); lanes past the last cell work on the last cell and
+  // store nothing
   const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (c >= d.ncell) return;
+  const bool valid = c < d.ncell;
+  const int64_t cv = valid ? c : d.ncell - 1;
+  const int64_t xc = cv % d.cx;
   double ih[3];
   load_ih(k, ih);
-  const int64_t b = cell_base_node(c, d);
+  const int64_t b = cell_base_node(cv, d);
   Edges Eu;
   {
     double v[8][3];
@@ -625,11 +681,11 @@ __global__ void __launch_bounds__(BLOCK, 2) grad_locals_kernel(const double* __r
     soft_vjp(Eu, er, BLOCK, k, G);
   }
   if (MODE == 2) {
-    adj_store(G, ih, loc, c, d.ncell);
+    adj_store(G, ih, loc, c, xc, valid, d);
     adj_zero(G);
   }
   if (BARRIER) barrier_vjp(Eu, k, G);
-  adj_store(G, ih, MODE == 2 ? loc2 : loc, c, d.ncell);
+  adj_store(G, ih, MODE == 2 ? loc2 : loc, c, xc, valid, d);
 }
 
 // GGN diagonal per cell (linearization.py:357-380; gen_triton_q1.py:447-515):
@@ -648,13 +704,15 @@ __device__ __forceinline__ void basis_grad(int a, const double s[3], const doubl
 
 __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restrict__ u, const double* __restrict__ k,
                                                            Dims d, double* __restrict__ loc) {
+  // every lane runs to the end (the locals' lane shuffle); lanes past the last cell store nothing
   const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-  if (c >= d.ncell) return;
+  const bool valid = c < d.ncell;
+  const int64_t cv = valid ? c : d.ncell - 1;
   double ih[3];
   load_ih(k, ih);
   const double mu = k[K_MU], nu = k[K_NU];
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
-  const int64_t b = cell_base_node(c, d);
+  const int64_t b = cell_base_node(cv, d);
   Edges Eu;
   {
     double v[8][3];
@@ -717,14 +775,15 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
       }
     }
   }
+  const int64_t xc = cv % d.cx;
 #pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int a = 0; a < 8; a += 2)
 #pragma unroll
-    for (int r = 0; r < 3; ++r) loc[(int64_t)(a * 3 + r) * d.ncell + c] = acc[a][r];
+    for (int r = 0; r < 3; ++r) locals_store_pair(acc[a][r], acc[a + 1][r], loc, a, r, c, xc, valid, d);
 }
 
-// out[n,r] = base[n,r] + sum_{a=0..7} loc[a*3+r][cell(n,a)] (+ loc2 likewise), fixed vertex order
-// (the reference gathers in the same order, gen_triton_q1.py:425-444). base may be NULL.
+// out[n,r] = base[n,r] + the cell contributions of loc (then of loc2) to node n in locals_gather's fixed order
+// (the reference gathers vertex by vertex, gen_triton_q1.py:425-444). base may be NULL.
 template <bool TWO>
 __global__ void __launch_bounds__(BLOCK) gather_kernel(const double* __restrict__ base,
                                                        const double* __restrict__ loc,
@@ -734,29 +793,13 @@ __global__ void __launch_bounds__(BLOCK) gather_kernel(const double* __restrict_
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nnode) return;
   const int64_t x = i % d.nx, y = (i / d.nx) % d.ny, z = i / (d.nx * d.ny);
-  double acc[3];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) acc[r] = base ? base[3 * i + r] : 0.0;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    const int64_t xa = x - (a & 1), ya = y - ((a >> 1) & 1), za = z - ((a >> 2) & 1);
-    if (xa < 0 || xa >= d.cx || ya < 0 || ya >= d.cy || za < 0 || za >= d.cz) continue;
-    const int64_t cell = (za * d.cy + ya) * d.cx + xa;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) acc[r] += loc[(int64_t)(a * 3 + r) * d.ncell + cell];
+  for (int r = 0; r < 3; ++r) {
+    double acc = base ? base[3 * i + r] : 0.0;
+    acc = locals_gather<true>(loc, r, x, y, z, d, acc);
+    if (TWO) acc = locals_gather<true>(loc2, r, x, y, z, d, acc);
+    out[3 * i + r] = acc;
   }
-  if (TWO) {
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int64_t xa = x - (a & 1), ya = y - ((a >> 1) & 1), za = z - ((a >> 2) & 1);
-      if (xa < 0 || xa >= d.cx || ya < 0 || ya >= d.cy || za < 0 || za >= d.cz) continue;
-      const int64_t cell = (za * d.cy + ya) * d.cx + xa;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) acc[r] += loc2[(int64_t)(a * 3 + r) * d.ncell + cell];
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < 3; ++r) out[3 * i + r] = acc[r];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1278,7 +1321,7 @@ __global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __rest
 
 // The energy gradient on the free nodes in one pass: for interior node f (trapezoid weight 1, so q = omega) the
 // total node term qd (W - F) b + lam q (u - ref), written as node_terms_kernel writes it, plus the eight cell
-// contributions in vertex order a = 0..7 as gather_kernel adds them, stored straight into the component-major free
+// contributions in the order gather_kernel adds them, stored straight into the component-major free
 // vector g[c F + f] (what restrict_kernel would make of the node field). Saves the node-term and restriction passes
 // over the whole node field.
 __global__ void __launch_bounds__(BLOCK) gather_free_kernel(const double* __restrict__ warped,
@@ -1301,19 +1344,14 @@ __global__ void __launch_bounds__(BLOCK) gather_free_kernel(const double* __rest
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double im = r * bgrad[3 * i + c];
-    double acc = im + lq * (u[3 * i + c] - ref[3 * i + c]);
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int64_t cell = ((z - ((a >> 2) & 1)) * d.cy + (y - ((a >> 1) & 1))) * d.cx + (x - (a & 1));
-      acc += loc[(int64_t)(a * 3 + c) * d.ncell + cell];
-    }
-    g[c * F + f] = acc;
+    const double acc = im + lq * (u[3 * i + c] - ref[3 * i + c]);
+    g[c * F + f] = locals_gather<false>(loc, c, x, y, z, d, acc);
   }
 }
 
 // The three outer gradients on the free nodes in one pass (the joint form): gd = image node term, gf = image +
-// barrier contributions, ge = total node term + soft contributions + barrier contributions, each gathered in vertex
-// order as gather_kernel does (soft before barrier for ge) and stored component-major.
+// barrier contributions, ge = total node term + soft contributions + barrier contributions, each gathered in the
+// order gather_kernel uses (soft before barrier for ge) and stored component-major.
 __global__ void __launch_bounds__(BLOCK) joint_free_kernel(const double* __restrict__ warped,
                                                            const double* __restrict__ fixed,
                                                            const double* __restrict__ dw,
@@ -1333,21 +1371,13 @@ __global__ void __launch_bounds__(BLOCK) joint_free_kernel(const double* __restr
   const double qd = dw ? q * dw[i] : q;
   const double lq = lam * q;
   const double r = qd * (warped[i] - fixed[i]);
-  int64_t cells[8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-    cells[a] = ((z - ((a >> 2) & 1)) * d.cy + (y - ((a >> 1) & 1))) * d.cx + (x - (a & 1));
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double im = r * bgrad[3 * i + c];
-    double outer = im;
+    const double outer = locals_gather<false>(bar, c, x, y, z, d, im);
     double total = im + lq * (u[3 * i + c] - ref[3 * i + c]);
-#pragma unroll
-    for (int a = 0; a < 8; ++a) outer += bar[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) total += soft[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) total += bar[(int64_t)(a * 3 + c) * d.ncell + cells[a]];
+    total = locals_gather<false>(soft, c, x, y, z, d, total);
+    total = locals_gather<false>(bar, c, x, y, z, d, total);
     gd[c * F + f] = im;
     gf[c * F + f] = outer;
     ge[c * F + f] = total;

@@ -51,8 +51,8 @@ TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_soft_energy": 986e6,
     "seqvp_q1_barrier_energy": 362e6,
     "seqvp_q1_gradient": 3666e6,
-    "seqvp_q1_gradient_free": 4344e6,
-    "seqvp_q1_joint": 6353e6,
+    "seqvp_q1_gradient_free": 3940e6,
+    "seqvp_q1_joint_free": 5973e6,
     "seqvp_jd_root": 361e6,
     "seqvp_certificate_screen": 283e6,
     "seqvp_spline_warp": 605e6,

@@ -163,7 +163,8 @@ def precision_form(args, pair, problem, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st, res, rec = preoptimized_solve(problem, pair.fixed, pair.moving, dev, cfg, max_outer=args.outer,
-                                      observer=log("preopt-route"), sampler=spline)
+                                      observer=log("preopt-route"), sampler=spline,
+                                      inner_config=PGConfig(jd_step_global=True))
     torch.cuda.synchronize()
     route_s = time.perf_counter() - t0
     if res is None:
@@ -172,7 +173,8 @@ def precision_form(args, pair, problem, dev):
     st64.set_state(theta=torch.zeros(st64.nfree, dtype=torch.float64, device=dev))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    res64 = solve_sequential(st64, SequentialConfig(), PGConfig(), observer=log("fp64-only"), max_outer=args.outer)
+    res64 = solve_sequential(st64, SequentialConfig(), PGConfig(jd_step_global=True), observer=log("fp64-only"),
+                             max_outer=args.outer)
     torch.cuda.synchronize()
     fp64_s = time.perf_counter() - t1
     levels = [dict(level=lv["level"], shape_zyx=lv["shape_zyx"], iterations=lv["iterations"],
@@ -362,7 +364,8 @@ def main():
                 print(f"{label} {k} done E {st.energy.total:.6e}", file=sys.stderr, flush=True)
     else:
         from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
-        inner_cfg = PGConfig(max_iterations=args.inner, fp32_screen=args.fp32_screen)
+        # the certified solver with the cubic JD-root bound on the device (T66, jd_step_global)
+        inner_cfg = PGConfig(max_iterations=args.inner, fp32_screen=args.fp32_screen, jd_step_global=True)
 
         def run(n, label):
             def obs(row):

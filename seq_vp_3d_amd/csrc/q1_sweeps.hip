@@ -507,6 +507,10 @@ __device__ __forceinline__ constexpr int vertex_k(int a, int j) {
 // by a lane shuffle and stored once: plane a (x bit 0) of cell c holds v_a(c) + v_{a+1}(c - 1) where c merges, and
 // plane a + 1 holds v_{a+1}(c) only for the cells whose successor does not merge (a wave's last lane, a row's last
 // cell); its other entries are never written or read. Halves the locals traffic of the sweeps and gathers.
+// Callers must map cell c to lane c & 63 (a block of a multiple of 64 lanes starting at a multiple of 64) and keep
+// every lane of the wave converged at locals_store_pair (no per-lane early return before it): the shuffle reads the
+// neighbouring lane's value, and a lane that left would leave a plane unwritten that the gather then reads.
+static_assert(BLOCK % 64 == 0, "the locals merge assumes lane = cell & 63");
 __device__ __forceinline__ bool locals_merge(int64_t c, int64_t x) { return (c & 63) != 0 && x != 0; }
 __device__ __forceinline__ bool locals_keep_odd(int64_t c, int64_t x, const Dims& d) {
   return ((c + 1) & 63) == 0 || x == d.cx - 1;

@@ -135,12 +135,18 @@ class JDStepMemory:
             cells, points = self.layout()
             try:
                 C = coefficients(cells, points)
-                info["points"] = len(cells)
                 least = None
-                for c in C:
-                    r = first_root_before(c, limit)
-                    if r is not None:
-                        least = r if least is None else min(least, r)
+                # points are counted per witness cell once its cubics are done, so a failed cubic leaves the
+                # count of the cells before it (jd_step.py:212-222)
+                ends = list(np.flatnonzero(np.diff(cells)) + 1) + [len(cells)]
+                start = 0
+                for end in ends:
+                    for c in C[start:end]:
+                        r = first_root_before(c, limit)
+                        if r is not None:
+                            least = r if least is None else min(least, r)
+                    info["points"] += end - start
+                    start = end
                 if least is not None:
                     info["root"] = least
                     tau = min(limit, safety * least)

@@ -373,12 +373,13 @@ def certify_transfer(st, seed, alphas=TRANSFER_ALPHAS):
 
 
 def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_outer=None, observer=None,
-                       sampler=None, report_errors=True, budget=None, on_state=None):
+                       sampler=None, report_errors=True, budget=None, on_state=None, inner_config=None):
     """The precision form: FP32 pyramid pre-optimisation, boundary taper, certified transfer, then the certified
     FP64 solve (solve_sequential) from the transferred state (the pre-optimizer route of
     scripts/run_stage_3d_e19_mri.py:930-960 and multiresolution.prepare_transfer_problem). The native solve keeps
     the zero-displacement baseline (multiresolution.py:496-502); `budget()` is its wall-budget predicate and
-    `on_state(st)` is called once the PairState exists (before any outer row reaches `observer`).
+    `on_state(st)` is called once the PairState exists (before any outer row reaches `observer`); `inner_config` is
+    the PGConfig of the certified solve (the native route's defaults when None).
     Returns (PairState, SequentialResult or None when no alpha certified, record)."""
     from .hotpath import PairState
     from .solver import PGConfig, SequentialConfig, solve_sequential
@@ -400,7 +401,7 @@ def preoptimized_solve(problem, fixed, moving, device="cuda", config=None, max_o
     if alpha is None:
         return st, None, record
     record["handoff_energy"] = st.energy.total
-    res = solve_sequential(st, SequentialConfig(), PGConfig(), observer=observer, max_outer=max_outer,
-                           budget=budget, baseline_at_zero=True)
+    res = solve_sequential(st, SequentialConfig(), PGConfig() if inner_config is None else inner_config,
+                           observer=observer, max_outer=max_outer, budget=budget, baseline_at_zero=True)
     record["solve_seconds"] = perf_counter() - t2
     return st, res, record

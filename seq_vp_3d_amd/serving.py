@@ -43,13 +43,15 @@ def footprint_bytes(shape_zyx):
 @dataclass
 class Request:
     """One registration: images (host float64 arrays or CPU tensors -- pinned ones upload asynchronously -- of the
-    same (nz, ny, nx) shape on a unit-spacing grid) and the number of certified outer steps to run."""
+    same (nz, ny, nx) shape on a unit-spacing grid), the number of certified outer steps to run and whether the
+    solver caps its steps by the device cubic JD root (T66)."""
     fixed: object
     moving: object
     max_outer: int = 10
     request_id: int = 0
     arrival: float = 0.
     recertify: bool = True
+    jd_step_global: bool = True
 
     @property
     def shape(self):
@@ -96,7 +98,8 @@ def run_request(req, device, stream=None, recert_workers=8):
         st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=device))
         t1 = time.perf_counter()
         secs["setup"] = t1 - started
-        res = solve_sequential(st, SequentialConfig(), PGConfig(), max_outer=req.max_outer)
+        res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=req.jd_step_global),
+                               max_outer=req.max_outer)
         t2 = time.perf_counter()
         secs["solve"] = t2 - t1
         theta = st.theta.cpu().numpy()

@@ -1,9 +1,10 @@
 """Device-resident certified PG solver: the inner projected-gradient loop and the outer proximal loop.
 
-Restates the decisions of the reference's default route -- solve_poisson_gradient with
-ConservativePGConfig3D(resolution_exit=True, jd_step_global=True) and the
-image_shift_ggn_equilibrated_jd_affine_v3 metric (T62 node measure with the T69 face/interior extension always
-on; jd_curl/three_d/solvers/poisson_gradient.py:400-576, 1090-1646), and
+Restates the decisions of the reference's native route -- solve_poisson_gradient with
+ConservativePGConfig3D(resolution_exit=True) and the image_shift_ggn_equilibrated_jd_affine_v3 metric (T62 node
+measure with the T69 face/interior extension always on; jd_curl/three_d/solvers/poisson_gradient.py:400-576,
+1090-1646; docs/usage.md:84-93), with the T66 global root bound as an option (jd_step_global, off by default as in
+ConservativePGConfig3D, poisson_gradient.py:143), and
 solve_sequential (jd_curl/three_d/sequential.py:151-689) -- on a PairState whose arrays never leave HBM.
 Only scalars cross PCIe. The first trial step of an inner iteration is the historical-witness prediction
 (jd_step.py:177-239), capped by the T66 global closed27 root when jd_step_global is set. A trial is valid only
@@ -43,7 +44,7 @@ class PGConfig:
     stall_steps: int = 5
     jd_step_safety: float = .9
     resolution_exit: bool = True
-    jd_step_global: bool = True
+    jd_step_global: bool = False         # T66 global closed27 root cap (poisson_gradient.py:143 default)
     shift_factor: float = 1.
     jd_affine_activation: float = .25
     metric_policy: str = T73_POLICY
@@ -194,6 +195,33 @@ def step_proposal(st, config, previous, products=None):
     return _protect(config, proposal), "bb2"
 
 
+def precision_stagnation_evidence(attempts, untruncated):
+    """Every attempt of a search whose first step was the untruncated proposal is a valid comparison that the
+    resolution cannot decide (energy_comparison.py:64-77)."""
+    if not attempts or not untruncated:
+        return False
+    if any(r["reason"] != "ENERGY_RESOLUTION" or not r["trial_valid"] or r["accepted"]
+           or r["energy_resolution"] is None for r in attempts):
+        return False
+    return all(r["actual_reduction"] is not None and np.isfinite(r["actual_reduction"])
+               and np.isfinite(r["energy_resolution"]) and r["energy_resolution"] >= 0
+               and abs(r["actual_reduction"]) <= r["energy_resolution"] for r in attempts)
+
+
+def failed_search_status(attempts):
+    """Status of a line search that accepted nothing, from the set of its reasons (poisson_gradient.py:1014-1025)."""
+    reasons = {r["reason"] for r in attempts}
+    if reasons == {"INVALID_DOMAIN"}:
+        return "domain_blocked"
+    if reasons == {"FEASIBILITY_UNCERTAIN"}:
+        return "feasibility_uncertain"
+    if reasons == {"JACOBIAN_VIOLATION"}:
+        return "feasibility_blocked"
+    if reasons and all(r.startswith("NONFINITE") for r in reasons):
+        return "nonfinite_evaluation"
+    return "line_search_failed"
+
+
 def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=None, memory=None):
     """One frozen-reference inner solve on the accepted state of `st` (poisson_gradient.py:1090-1646). `memory` is
     the JD witness memory of the level (a fresh one when None, poisson_gradient.py:1130-1136)."""
@@ -295,6 +323,7 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
         untruncated = tau == proposal
         exit_floor = 15 * EPS * st.energy.total if config.resolution_exit else None
         global_attempts = 0
+        first_attempt = len(history)
         accepted = False
         for attempt in range(config.max_backtracks + 1):
             if budget is not None and budget():
@@ -325,16 +354,23 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                     accepted = certified > 0 and certified >= required
                     reason = ("ACCEPTED" if accepted else "ENERGY_RESOLUTION" if abs(actual) <= resolution
                               else "NO_STRICT_DECREASE" if actual < 0 else "ARMIJO_REJECTED")
-            exit_info = None
+            exit_info = next_prediction = None
             if (config.resolution_exit and not accepted and not reason.startswith("NONFINITE")
                     and global_attempts + 1 <= config.max_backtracks):
-                next_tau = .5 * tau
+                if reason == "JACOBIAN_VIOLATION":
+                    # the memory now holds this trial's witness: the next step is also capped by its prediction
+                    # (poisson_gradient.py:1503-1508)
+                    repredicted, next_prediction = memory.predict(st.witness_coefficients, tau, config.jd_step_safety)
+                    next_tau = min(.5 * tau, repredicted)
+                else:
+                    next_tau, next_prediction = .5 * tau, prediction
                 decrease = next_tau * abs(slope)
                 exit_info = dict(next_tau=next_tau, predicted_decrease=decrease, resolution_floor=exit_floor,
                                  stop=bool(decrease <= exit_floor))
             row = dict(iteration=accepted_steps, attempt=attempt + 1, accepted=bool(accepted), reason=reason, tau=tau,
                        step_proposal=proposal, step_rule=rule, untruncated=untruncated, domain_step_bound=bound,
-                       gradient_norm=G, gradient_dot_direction=slope, actual_reduction=actual,
+                       gradient_norm=G, gradient_dot_direction=slope, trial_valid=bool(tr.ok and not tr.unproven_cells),
+                       actual_reduction=actual,
                        energy_resolution=resolution, certified_reduction=certified, energy_base=st.energy.total,
                        energy_trial=tr.energy.total if tr.energy is not None else None,
                        unproven_cells=tr.unproven_cells, certificate=certificate_record(tr.certificate),
@@ -356,21 +392,23 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             if global_attempts > config.max_backtracks:
                 break
             if exit_info is not None:
-                tau = exit_info["next_tau"]
+                tau, prediction = exit_info["next_tau"], next_prediction
                 if exit_info["stop"]:
                     break
+            elif reason == "JACOBIAN_VIOLATION":
+                repredicted, prediction = memory.predict(st.witness_coefficients, tau, config.jd_step_safety)
+                tau = min(.5 * tau, repredicted)
             else:
                 tau *= .5
         if not accepted:
-            last = history[-1]["reason"]
-            status = ("precision_stagnated" if last in ("ENERGY_RESOLUTION", "NO_STRICT_DECREASE")
-                      else "feasibility_uncertain" if last == "FEASIBILITY_UNCERTAIN"
-                      else "trial_budget_exhausted")
-            return finish(status, last)
+            attempts = history[first_attempt:]
+            status = ("precision_stagnated" if precision_stagnation_evidence(attempts, untruncated)
+                      else failed_search_status(attempts))
+            return finish(status, attempts[-1]["reason"])
 
 
 def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), budget=None, observer=None,
-                     max_outer=None, baseline_at_zero=False):
+                     max_outer=None, baseline_at_zero=False, inner_observer=None):
     """Outer proximal loop (sequential.py:441-689): inner solve from the committed state, stable E and F
     comparisons against the start, commit, self-reference refresh. With baseline_at_zero (the native level of
     the four-level route, multiresolution.py:496-502) G0, its threshold and the baseline MSE are those of u = 0
@@ -414,11 +452,13 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
         st.outer_cache = None
         eta = config.eta0 if G0 == 0 else config.eta0 * np.sqrt(min(GF, G0) / G0)
         target = float(eta * GF)
-        inner = solve_poisson_gradient(st, target, inner_config, budget=budget, memory=memory)
+        inner = solve_poisson_gradient(st, target, inner_config, budget=budget, memory=memory,
+                                       observer=inner_observer)
         eligible = inner.status in ("converged_gradient", "max_iterations", "stagnated")
-        accepted, reason = False, "INNER_FAILED_NOT_COMMITTED"
         total = fixed = None
         GFa = GDa = GEa = relative = None
+        # an inner solve without an accepted step hands back the start itself: no progress, nothing to compare
+        cert, same = False, True
         if inner.accepted_steps > 0:
             cached = getattr(st, "outer_cache", None)
             GFa, GDa, GEa = cached if cached is not None else st.outer_norms()
@@ -429,9 +469,9 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
             total, fixed = (-dE, resE), (-dF, resF)
             cert = (-dE - resE) > 0 and (-dF - resF) > 0
             same = bool(torch.equal(st.theta, start["theta"]))
-            accepted = bool(eligible and cert and not same)
-            reason = ("ACCEPTED" if accepted else "INNER_FAILED_NOT_COMMITTED" if not eligible
-                      else "NO_PROGRESS" if same else "DESCENT_CERTIFICATE_FAILED")
+        accepted = bool(eligible and cert and not same)
+        reason = ("ACCEPTED" if accepted else "INNER_FAILED_NOT_COMMITTED" if not eligible
+                  else "NO_PROGRESS" if same else "DESCENT_CERTIFICATE_FAILED")
         valid = inner.accepted_steps > 0
         se, ce = start["energy"], st.energy
         cc = st.certificate if valid else None
@@ -474,7 +514,7 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
                 return finish("outer_time_budget", reason)
             if inner.status == "precision_stagnated":
                 return finish("outer_precision_stagnated", reason)
-            if inner.status in ("domain_blocked", "feasibility_uncertain"):
+            if inner.status in ("domain_blocked", "feasibility_blocked", "feasibility_uncertain"):
                 return finish("outer_" + inner.status, inner.reason)
             if not eligible:
                 return finish("outer_inner_failed", inner.status + ": " + inner.reason)

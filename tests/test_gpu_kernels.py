@@ -137,7 +137,7 @@ def test_jd_root(gpu_device):
         if np.isinf(want[0]):
             assert np.isinf(got[0])
         else:
-            assert abs(got[0] - want[0]) <= 1e-9 * want[0], (limit, got, want)
+            assert abs(got[0] - want[0]) <= 1e-12 * want[0], (limit, got, want)
     # the direction as a free-node vector: bitwise the root of its node field
     pf = Q.restrict_free(v, og)
     vn = Q.inject_free(pf, og)

@@ -1,4 +1,5 @@
-"""The device solver against the oracle-backed CPU twin, step by step (same decisions, same tau)."""
+"""The device solver against the oracle's restatement of the reference's driver and against the oracle-backed CPU
+twin, step by step (same decisions, same tau)."""
 import numpy as np
 import pytest
 
@@ -9,12 +10,18 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def test_device_solver_matches_cpu_twin_stepwise(gpu_device):
+@pytest.mark.parametrize("jd_step_global,initial_step", [(False, 1.), (True, 1.), (False, 10.)])
+def test_device_solver_matches_the_reference_flow(gpu_device, jd_step_global, initial_step):
+    """The device driver on PairState against the oracle's independent restatement of the reference's inner PG
+    solve and outer loop (oracle/driver.py): every attempt's decision, tau and trial energy, and every outer row, to
+    1e-12 relative, over 4 outer steps."""
+    from test_driver_oracle import compare_runs
+    from oracle import driver as od
     from seq_vp_3d_amd.hotpath import PairState
     from seq_vp_3d_amd.problem import Q1Problem, native_weights
     from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
     from seq_vp_3d_amd.synthetic import PairSpec, make_pair
-    print("case solver stepwise 14x15x16", flush=True)
+    print("case solver vs driver oracle 14x15x16", jd_step_global, initial_step, flush=True)
 
     def sampler(img, grid, pts):
         s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
@@ -23,22 +30,19 @@ def test_device_solver_matches_cpu_twin_stepwise(gpu_device):
     spec = PairSpec((14, 15, 16), 6, (2., 4.), 3, .6, .01, 3)
     pair = make_pair(spec, "cpu", sampler)
     w = native_weights(spec.shape)
-    cpu = CpuPairState(spec.shape, (1., 1., 1.), w, pair.fixed.numpy(), pair.moving.numpy())
-    theta0 = Q.restrict_free(.5 * pair.displacement.numpy(), cpu.grid)
-    cpu.set_state(theta0)
+    theta0 = Q.restrict_free(.5 * pair.displacement.numpy(), Q.Grid(spec.shape, (1., 1., 1.)))
+    outer = 4
+    opair = od.Pair(spec.shape, (1., 1., 1.), w, pair.fixed.numpy(), pair.moving.numpy())
+    want = od.solve_sequential(opair, theta0, od.SequentialConfig(max_iterations=outer),
+                               od.PGConfig(jd_step_global=jd_step_global, initial_step=initial_step))
     gpu = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
     gpu.set_state(theta=torch.tensor(theta0, device=gpu_device))
-    rows_c, rows_g = [], []
-    rc = solve_sequential(cpu, SequentialConfig(), PGConfig(), max_outer=2, observer=rows_c.append)
-    rg = solve_sequential(gpu, SequentialConfig(), PGConfig(), max_outer=2, observer=rows_g.append)
-    assert rc.status == rg.status and rc.outer_accepted == rg.outer_accepted == 2
-    for a, b in zip(rows_c, rows_g):
-        assert a["reason"] == b["reason"] and a["inner_accepted_steps"] == b["inner_accepted_steps"]
-        assert a["inner_trials"] == b["inner_trials"]
-        for key in ("E_start", "E_candidate", "GF_candidate", "GE_candidate"):
-            assert abs(a[key] - b[key]) <= 1e-10 * abs(a[key]), key
-    np.testing.assert_allclose(gpu.theta.cpu().numpy(), cpu.theta.numpy(), rtol=0,
-                               atol=1e-10 * np.abs(cpu.theta.numpy()).max())
+    rows, inner_rows = [], []
+    res = solve_sequential(gpu, SequentialConfig(), PGConfig(jd_step_global=jd_step_global, initial_step=initial_step),
+                           max_outer=outer, observer=rows.append, inner_observer=inner_rows.append)
+    compare_runs(want, want.inner_history, res, rows, inner_rows)
+    np.testing.assert_allclose(gpu.theta.cpu().numpy(), want.state.theta, rtol=0,
+                               atol=1e-12 * np.abs(want.state.theta).max())
 
 
 def test_fp32_screen_keeps_the_fp64_decisions(gpu_device):
@@ -111,8 +115,42 @@ def test_witness_memory_and_prediction_match_cpu_twin(gpu_device, jd_step_global
         assert pa["witnesses"] == pb["witnesses"] and pa["points"] == pb["points"]
         assert (pa["root"] is None) == (pb["root"] is None)
         if pa["root"] is not None:
-            assert abs(pa["root"] - pb["root"]) <= 1e-10 * pa["root"]
+            assert abs(pa["root"] - pb["root"]) <= 1e-12 * pa["root"]
     if not jd_step_global:
         assert any(r["reason"] == "JACOBIAN_VIOLATION" for r in rows_g)
         assert len(mem_g.points) >= 1
     assert list(mem_c.points) == list(mem_g.points)
+
+
+def test_root_launch_then_witness_cubics_then_read(gpu_device):
+    """The solver enqueues the T66 root sweep, evaluates the witness cubics (which write the direction's node field
+    on the same stream) and only then reads the root: the value is the synchronous root's, bit for bit."""
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.jd_step import JDStepMemory
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print("case root launch ordering", flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    spec = PairSpec((14, 15, 16), 6, (2., 4.), 3, .6, .01, 3)
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    st = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+    st.set_state(theta=torch.tensor(Q.restrict_free(.5 * pair.displacement.numpy(), Q.Grid(spec.shape, (1., 1., 1.))),
+                                    device=gpu_device))
+    st.refresh_reference()
+    st.gradient()
+    st.metric(shift_factor=1., activation=.25, extension=True)
+    st.direction()
+    mem = JDStepMemory()
+    mem.bind(pair.grid)
+    mem.witness(40, (.3, .6, .2))
+    mem.witness(700, (.5, .5, .5))
+    for limit in (.5, 3., 20.):
+        want = st.jd_bound(limit)
+        pending = st.jd_bound_launch(limit)
+        mem.predict(st.witness_coefficients, limit, .9)
+        assert st.jd_bound_read(pending) == want, limit

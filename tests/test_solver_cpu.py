@@ -96,3 +96,89 @@ def test_zero_baseline_and_reported_mse(pair):
     assert res_zero.mse_ratio == res_zero.final_mse / res_zero.native_mse
     # the state is back where it started
     np.testing.assert_array_equal(st.theta.numpy(), _state(spec, p).theta.numpy())
+
+
+class _Scripted:
+    """A state whose trials are scripted: the line search meets a Jacobian violation at the first step, and the
+    witness cubic of the memory has its first root at `root` (c0 + c1 s with c0 = 1, c1 = -1/root)."""
+
+    def __init__(self, root, outcomes):
+        from seq_vp_3d_amd.hotpath import Energies
+        self.grid = type("G", (), dict(shape=(4, 4, 4), spacing=(1., 1., 1.), n_cells=27))()
+        self.energy = Energies(1., 0., 0., 0., 0.)
+        self.root, self.outcomes, self.taus = root, list(outcomes), []
+        self.theta = self.theta_t = torch.zeros(24, dtype=torch.float64)
+
+    def gradient(self):
+        pass
+
+    def gradient_norm(self):
+        return 1.
+
+    def outer_norms(self):
+        return 1., 1., 1.
+
+    def metric(self, **kw):
+        pass
+
+    def direction(self, fp32=False):
+        pass
+
+    def direction_scalars(self, previous, norm=False, ratio=False):
+        return -1., None, 1e300, 1., None
+
+    def witness_coefficients(self, cells, points):
+        return np.tile([1., -1. / self.root, 0., 0.], (len(cells), 1))
+
+    def trial(self, tau):
+        from seq_vp_3d_amd.hotpath import Energies, TrialResult
+        self.taus.append(tau)
+        kind = self.outcomes.pop(0)
+        if kind == "fold":
+            return TrialResult(False, "JACOBIAN_VIOLATION", None, witness=(5, (.5, .5, .5)))
+        return TrialResult(True, "", Energies(1. - tau, 0., 0., 0., 0.), -tau, -tau)
+
+    def step_ratio_launch(self, a, b):
+        pass
+
+    def read_step_ratio(self):
+        return 1.
+
+    def promote(self, energy, keep_history=False):
+        self.energy = energy
+        return {}
+
+
+@pytest.mark.parametrize("resolution_exit", [True, False])
+def test_violation_repredicts_the_next_step(resolution_exit):
+    """After a JACOBIAN_VIOLATION the memory holds the trial's witness and the next step is
+    min(tau / 2, safety * its first root) (poisson_gradient.py:1503-1508, 1608-1610), not a plain halving."""
+    from seq_vp_3d_amd.jd_step import JDStepMemory
+    st = _Scripted(root=.1, outcomes=["fold", "ok"])
+    mem = JDStepMemory()
+    mem.bind(st.grid)
+    cfg = PGConfig(max_iterations=1, initial_step=1., resolution_exit=resolution_exit)
+    inner = solve_poisson_gradient(st, 0., cfg, memory=mem)
+    assert inner.accepted_steps == 1
+    assert st.taus[0] == 1. and st.taus[1] == pytest.approx(.9 * .1, rel=1e-15)
+    # with the root beyond tau / 2 the halving binds
+    st = _Scripted(root=.9, outcomes=["fold", "ok"])
+    mem = JDStepMemory()
+    mem.bind(st.grid)
+    solve_poisson_gradient(st, 0., cfg, memory=mem)
+    assert st.taus[1] == .5
+
+
+def test_failed_search_status_follows_the_reasons():
+    from seq_vp_3d_amd.solver import failed_search_status, precision_stagnation_evidence
+    row = lambda reason, **kw: dict(dict(reason=reason, trial_valid=True, accepted=False, energy_resolution=1e-12,
+                                         actual_reduction=5e-13), **kw)
+    assert failed_search_status([row("JACOBIAN_VIOLATION")] * 3) == "feasibility_blocked"
+    assert failed_search_status([row("INVALID_DOMAIN")]) == "domain_blocked"
+    assert failed_search_status([row("FEASIBILITY_UNCERTAIN")] * 2) == "feasibility_uncertain"
+    assert failed_search_status([row("ARMIJO_REJECTED"), row("JACOBIAN_VIOLATION")]) == "line_search_failed"
+    rows = [row("ENERGY_RESOLUTION")] * 3
+    assert precision_stagnation_evidence(rows, True)
+    assert not precision_stagnation_evidence(rows, False)             # a truncated first step is no evidence
+    assert not precision_stagnation_evidence(rows + [row("NO_STRICT_DECREASE")], True)
+    assert not precision_stagnation_evidence([row("ENERGY_RESOLUTION", actual_reduction=2e-12)], True)

@@ -44,6 +44,16 @@ int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base,
                          int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                          size_t workspace_bytes, void* stream);
 
+/* seqvp_q1_soft_energy without a workspace and without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The block partials live in a fixed 4 MiB device array of the library (one per GPU, resolved on
+ * the first call on that device and held until the process ends), so grids are limited to (nx-1)(ny-1)(nz-1) <=
+ * 22,369,536 cells (87,381 blocks of 256 cells); a larger grid returns SEQVP_EWORKSPACE and launches nothing.
+ * Same kernels, same results (bit for bit) as seqvp_q1_soft_energy. Calls that overlap on one GPU (several
+ * streams or host threads at once) share that array and race; serialise them. */
+int seqvp_q1_soft_energy_noalloc(const double* u, const double* ref, const double* base, const double* k,
+                                 int64_t nx, int64_t ny, int64_t nz, double* out, void* stream);
+
 /* Closed27 localized barrier energy (replaces FusedQ1.barrier_energy, triton_q1.py:680-719;
  * barrier.py:19-36, energy_comparison.py:14-46).  out[0] = w sum psi(J), out[1] = min J,
  * out[2] = UNWEIGHTED sum of stable psi(J)-psi(Ja) vs base (the caller multiplies by w, as

@@ -8,6 +8,8 @@
 // stores; x-neighbour pairs in one wave are added by a lane shuffle and stored once), and one lane
 // per node adds them in a fixed order after the node base -- no atomics, so every result is
 // bit-reproducible.
+#include <mutex>
+
 #include "q1_common.h"
 #include "jd_extension.h"
 #include "abi_util.h"
@@ -1459,6 +1461,58 @@ extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const do
   const int64_t nb = cdiv(d.ncell, BLOCK);
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)ws;
+  if (base)
+    hipLaunchKernelGGL((soft_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
+  else
+    hipLaunchKernelGGL((soft_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+// ---- allocation-free soft energy ---------------------------------------------------------------------------------
+// The block partials of seqvp_q1_soft_energy_noalloc live in this fixed device array (one copy per GPU, held until
+// the process ends). Host code never uses its name as an address: each device's address comes from
+// hipGetSymbolAddress on that device, resolved on the entry point's first call there. Every block writes its whole
+// partial slot before the reduction reads it, so nothing needs clearing between calls.
+constexpr int64_t NOALLOC_PARTIAL_DOUBLES = 1 << 19;      // 4 MiB: 6 doubles per 256-cell block
+static __device__ __align__(256) double scratch[NOALLOC_PARTIAL_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double* scratch_address[NOALLOC_MAX_DEVICES] = {};
+
+// The device `stream` runs on (the current device for the null stream) and that device's scratch address.
+int noalloc_scratch(hipStream_t stream, double** out) {
+  int current = 0;
+  if (hipGetDevice(&current) != hipSuccess) return SEQVP_ELAUNCH;
+  int device = current;
+  if (stream && hipStreamGetDevice(stream, &device) != hipSuccess) return SEQVP_EINVAL;
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return SEQVP_EINVAL;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[device]) {
+    if (device != current && hipSetDevice(device) != hipSuccess) return SEQVP_ELAUNCH;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (device != current) hipSetDevice(current);
+    if (e != hipSuccess || !p) return SEQVP_ELAUNCH;
+    scratch_address[device] = (double*)p;
+  }
+  *out = scratch_address[device];
+  return SEQVP_OK;
+}
+}  // namespace
+
+extern "C" int seqvp_q1_soft_energy_noalloc(const double* u, const double* ref, const double* base, const double* k,
+                                            int64_t nx, int64_t ny, int64_t nz, double* out, void* stream) {
+  if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  if (nb * 6 > NOALLOC_PARTIAL_DOUBLES) return SEQVP_EWORKSPACE;     // grid beyond the fixed scratch
+  hipStream_t s = (hipStream_t)stream;
+  double* part = nullptr;
+  const int rc = noalloc_scratch(s, &part);
+  if (rc != SEQVP_OK) return rc;
   if (base)
     hipLaunchKernelGGL((soft_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
   else

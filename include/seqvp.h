@@ -203,9 +203,11 @@ int seqvp_spline_warp(const double* u, const double* coeff, int64_t cz, int64_t 
                       double ox, double oy, double oz, double hx, double hy, double hz, int64_t nx, int64_t ny,
                       int64_t nz, double* value, double* grad, int64_t* status, void* stream);
 
-/* Same sampler at arbitrary points (npts, 3) xyz; status[0] counts points outside the closed box
- * [lox,hix]x[loy,hiy]x[loz,hiz] (their value/gradient are NaN). */
-int seqvp_spline_points(const double* points, int64_t npts, const double* coeff, int64_t cz, int64_t cy,
+/* Same sampler at arbitrary points (npts, 3) xyz (replaces TorchSpline.sample, backends/spline.py:81-127); status[0]
+ * counts points outside the closed box [lox,hix]x[loy,hiy]x[loz,hiz], and, where `strict` (npts bytes, NULL = none)
+ * is nonzero, outside the open box; their value/gradient are NaN. */
+int seqvp_spline_points(const double* points, int64_t npts, const uint8_t* strict, const double* coeff, int64_t cz,
+                        int64_t cy,
                         int64_t cx, const double* tz, int64_t ntz, const double* ty, int64_t nty,
                         const double* tx, int64_t ntx, double lox, double hix, double loy, double hiy,
                         double loz, double hiz, double* value, double* grad, int64_t* status, void* stream);

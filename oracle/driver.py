@@ -236,6 +236,10 @@ class Pair:
                 bound = min(bound, float(np.min((axis[0] - pts[..., r][neg]) / v[r][neg])))
         return bound
 
+    def global_root(self, state, p, limit):
+        """global_root_cpu (global_jd_step.py:107-129): (first root in s-units or inf, invalid base points, points)."""
+        return ojr.global_root(state.u, Q.inject_free(p, self.grid), self.grid, self.D, self.delta, limit)
+
     def step_ratio(self, delta, theta):
         """step_ratio (solvers/common.py:70-81)."""
         length = float(np.linalg.norm([ax[-1] - ax[0] for ax in self.grid.axes]))
@@ -355,7 +359,7 @@ def solve_poisson_gradient(pair, base, ref, target, config, memory, observer=Non
         if config.jd_step_global:
             # _global_prediction (:708-735)
             try:
-                s, invalid, points = ojr.global_root(base.u, v, grid, pair.D, pair.delta, tau)
+                s, invalid, points = pair.global_root(base, p, tau)
                 found = None if not np.isfinite(s) else s * tau
                 prediction["global"] = dict(root=found, invalid_base_points=invalid, points=points)
                 if found is not None and found != 0:

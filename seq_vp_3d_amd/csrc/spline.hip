@@ -166,15 +166,18 @@ __global__ void __launch_bounds__(256) spline_warp_kernel(const double* __restri
 }
 
 __global__ void __launch_bounds__(256) spline_points_kernel(const double* __restrict__ pts, int64_t npts,
-                                                            SplineArgs s, double lox, double hix, double loy,
-                                                            double hiy, double loz, double hiz,
-                                                            double* __restrict__ value, double* __restrict__ grad,
+                                                            const uint8_t* __restrict__ strict, SplineArgs s,
+                                                            double lox, double hix, double loy, double hiy,
+                                                            double loz, double hiz, double* __restrict__ value,
+                                                            double* __restrict__ grad,
                                                             unsigned long long* __restrict__ status) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int bad = 0;
   if (i < npts) {
     const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
-    const bool ok = px >= lox && px <= hix && py >= loy && py <= hiy && pz >= loz && pz <= hiz;
+    bool ok = px >= lox && px <= hix && py >= loy && py <= hiy && pz >= loz && pz <= hiz;
+    // a strict query must lie in the open support (TorchSpline.sample's strict mask, backends/spline.py:81-96)
+    if (strict && strict[i]) ok = ok && px > lox && px < hix && py > loy && py < hiy && pz > loz && pz < hiz;
     bad = ok ? 0 : 1;
     double v = __builtin_nan(""), g[3] = {__builtin_nan(""), __builtin_nan(""), __builtin_nan("")};
     if (ok) spline_eval(s, px, py, pz, &v, g, grad != nullptr);
@@ -216,7 +219,8 @@ extern "C" int seqvp_spline_warp(const double* u, const double* coeff, int64_t c
   return launch_status();
 }
 
-extern "C" int seqvp_spline_points(const double* points, int64_t npts, const double* coeff, int64_t cz, int64_t cy,
+extern "C" int seqvp_spline_points(const double* points, int64_t npts, const uint8_t* strict, const double* coeff,
+                                   int64_t cz, int64_t cy,
                                    int64_t cx, const double* tz, int64_t ntz, const double* ty, int64_t nty,
                                    const double* tx, int64_t ntx, double lox, double hix, double loy, double hiy,
                                    double loz, double hiz, double* value, double* grad, int64_t* status,
@@ -226,7 +230,7 @@ extern "C" int seqvp_spline_points(const double* points, int64_t npts, const dou
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(copy_or_zero_kernel, dim3(1), dim3(256), 0, s, nullptr, (double*)status, 1);
   if (npts == 0) return launch_status();
-  hipLaunchKernelGGL(spline_points_kernel, dim3((npts + 255) / 256), dim3(256), 0, s, points, npts, a, lox, hix, loy,
-                     hiy, loz, hiz, value, grad, (unsigned long long*)status);
+  hipLaunchKernelGGL(spline_points_kernel, dim3((npts + 255) / 256), dim3(256), 0, s, points, npts, strict, a, lox,
+                     hix, loy, hiy, loz, hiz, value, grad, (unsigned long long*)status);
   return launch_status();
 }

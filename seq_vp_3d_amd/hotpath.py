@@ -239,7 +239,7 @@ class PairState:
 
     def direction(self, fp32=False):
         """p = -S M^-1 (S g)."""
-        return self.poisson.apply_equilibrated(self.scale, self.g, out=self.p, fp32=fp32)
+        return self.poisson.equilibrated_direction(self.scale, self.g, out=self.p, fp32=fp32)
 
     def jd_bound(self, limit):
         return self.root.from_free(self.u, self.p, limit)

@@ -120,7 +120,44 @@ class DevicePoisson:
         """The same solve with the transforms and the division in float32 (torch_ops.py:117-136)."""
         return self._run("seqvp_dst_poisson_f32", rhs.reshape(-1), None, False, out)
 
-    def apply_equilibrated(self, scale, gradient, out=None, fp32=False):
-        """-(s * M^-1 (s * g)): the equilibrated PG direction (hybrid.py:230-286) in one call."""
+    def equilibrated_direction(self, scale, gradient, out=None, fp32=False):
+        """-(s * M^-1 (s * g)) for device vectors, in one launch: the equilibrated PG direction of the device
+        solver (the pre-scale, the solve and the negated post-scale of hybrid.py:230-286 fused)."""
         name = "seqvp_dst_poisson_f32" if fp32 else "seqvp_dst_poisson"
         return self._run(name, gradient.reshape(-1), scale.reshape(-1), True, out)
+
+    # ---- the host-array protocol of the reference's device Poisson (_HostPoisson, hybrid.py:224-286) ----------
+    def _device(self, a):
+        if isinstance(a, torch.Tensor):
+            return a.to(device=self.device, dtype=torch.float64).reshape(-1).contiguous()
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1), device=self.device)
+
+    def prepare_equilibrated(self, scale, gradient):
+        """The operands of apply_equilibrated: (scale, gradient) on the device (host arrays are uploaded)
+        (hybrid.py:241-262). A nonfinite s * g shows as a nonfinite direction, which apply_equilibrated hands back
+        as its ('x', ...) result."""
+        s, g = self._device(scale), self._device(gradient)
+        if s.numel() != self.n or g.numel() != self.n:
+            raise InputValidationError("scale and gradient must be (3F,) vectors")
+        return s, g
+
+    def apply_equilibrated(self, prepared, between=None):
+        """(direction, True) with direction = -(s * M^-1 (s * g)) as a host array when every value is finite, else
+        ('x', M^-1 (s * g) as a host array) for the caller's own tail (hybrid.py:264-286). `between` is host work run
+        once while the device computes."""
+        s, g = prepared
+        out = self.equilibrated_direction(s, g)
+        if between is not None:
+            between()
+        host = out.cpu().numpy()
+        if np.all(np.isfinite(host)):
+            return host, True
+        pre = (s.cpu().numpy() * g.cpu().numpy())
+        return "x", self.apply(torch.as_tensor(pre, device=self.device)).cpu().numpy()
+
+
+class TorchPoisson(DevicePoisson):
+    """DevicePoisson with the reference TorchPoisson's argument order (torch_ops.py:107-111): (device, inverse)."""
+
+    def __init__(self, device, inverse):
+        super().__init__(inverse, getattr(device, "device", device))

@@ -113,16 +113,33 @@ class DeviceSpline:
                   _lib.stream_handle(self.device))
         return value, (gradient if with_gradient else None), self.status
 
-    def sample(self, points, *, with_gradient=True):
-        """Value and xyz gradient at (..., 3) points; raises if a point leaves the closed support."""
-        if points.dtype != torch.float64 or points.shape[-1] != 3:
-            raise InputValidationError("points must be float64 (..., 3)")
+    def sample(self, points, *, chunk_size=65536, strict=None, with_gradient=True):
+        """Value and xyz gradient at (..., 3) device points (TorchSpline.sample, backends/spline.py:81-127): every
+        point must lie in the closed support and, where the boolean mask `strict` (the query shape) is set, in the
+        open support, else InputValidationError; the queries run in launches of at most `chunk_size` points."""
+        if (not isinstance(points, torch.Tensor) or points.dtype != torch.float64 or points.shape[-1:] != (3,)
+                or points.device != self.device):
+            raise InputValidationError("spline requires explicit device float64 xyz points")
+        if isinstance(chunk_size, bool) or not isinstance(chunk_size, int) or chunk_size < 1:
+            raise InputValidationError("positive integer spline chunk required")
         flat = points.reshape(-1, 3).contiguous()
         n = flat.shape[0]
+        mask = None
+        if strict is not None:
+            if (not isinstance(strict, torch.Tensor) or strict.dtype != torch.bool
+                    or tuple(strict.shape) != tuple(points.shape[:-1]) or strict.device != self.device):
+                raise InputValidationError("strict query mask mismatch")
+            mask = strict.reshape(-1).contiguous().view(torch.uint8)
         value = torch.empty(n, dtype=torch.float64, device=self.device)
         gradient = torch.empty((n, 3), dtype=torch.float64, device=self.device) if with_gradient else None
-        _lib.call("seqvp_spline_points", _lib.ptr(flat), n, *self._spline_args(), *self.box, _lib.ptr(value),
-                  _lib.ptr(gradient), _lib.ptr(self.status), _lib.stream_handle(self.device))
-        if int(self.status.item()):
+        chunks = range(0, n, chunk_size)
+        status = torch.zeros(max(1, len(chunks)), dtype=torch.int64, device=self.device)
+        for k, start in enumerate(chunks):
+            end = min(n, start + chunk_size)
+            _lib.call("seqvp_spline_points", _lib.ptr(flat[start:end]), end - start,
+                      None if mask is None else _lib.ptr(mask[start:end]), *self._spline_args(), *self.box,
+                      _lib.ptr(value[start:end]), None if gradient is None else _lib.ptr(gradient[start:end]),
+                      _lib.ptr(status[k:k + 1]), _lib.stream_handle(self.device))
+        if int(status.sum().item()):
             raise InputValidationError("spline query outside permitted support")
         return value.reshape(points.shape[:-1]), (gradient.reshape(points.shape) if with_gradient else None)

@@ -211,6 +211,17 @@ def test_spline_warp_and_points(gpu_device):
     v, g = sp.sample(dev(np.ascontiguousarray(pts), gpu_device))
     wv, wg = osp.sample(ref, pts)
     assert rel(v.cpu().numpy(), wv) <= 1e-12 and rel(g.cpu().numpy(), wg) <= 1e-12
+    # TorchSpline.sample's keywords (backends/spline.py:81-96): chunks give the same values; a strict mask refuses
+    # a masked point on the support's end and accepts it unmasked
+    v2, g2 = sp.sample(dev(np.ascontiguousarray(pts), gpu_device), chunk_size=97)
+    assert torch.equal(v2, v) and torch.equal(g2, g)
+    edge = np.ascontiguousarray(np.stack([np.full(4, ax[0][0]), np.full(4, ax[1][3]), np.full(4, ax[2][2])], -1))
+    mask = torch.tensor([False, False, True, False], device=gpu_device)
+    with pytest.raises(Exception, match="outside permitted support"):
+        sp.sample(dev(edge, gpu_device), chunk_size=3, strict=mask)
+    sp.sample(dev(edge, gpu_device), strict=torch.zeros(4, dtype=torch.bool, device=gpu_device))
+    with pytest.raises(Exception, match="chunk"):
+        sp.sample(dev(edge, gpu_device), chunk_size=0)
 
 
 @pytest.mark.parametrize("fp32", [0, 1])
@@ -245,10 +256,21 @@ def test_dst_poisson(gpu_device, shape):
     s = rng.uniform(.5, 2., size=g.size)
     got = P.apply(dev(g, gpu_device)).cpu().numpy()
     assert rel(got, opo.apply(g, den)) <= 1e-12
-    got = P.apply_equilibrated(dev(s, gpu_device), dev(g, gpu_device)).cpu().numpy()
+    got = P.equilibrated_direction(dev(s, gpu_device), dev(g, gpu_device)).cpu().numpy()
     assert rel(got, -(s * opo.apply(s * g, den))) <= 1e-12
-    got32 = P.apply_equilibrated(dev(s, gpu_device), dev(g, gpu_device), fp32=True).cpu().numpy()
+    got32 = P.equilibrated_direction(dev(s, gpu_device), dev(g, gpu_device), fp32=True).cpu().numpy()
     assert rel(got32, -(s * opo.apply(s * g, den))) <= 2e-5
+    # the reference's host-array protocol (hybrid.py:224-286) with TorchPoisson's argument order (torch_ops.py:107)
+    from seq_vp_3d_amd.poisson import TorchPoisson
+    T = TorchPoisson(gpu_device, inv)
+    seen = []
+    d, ok = T.apply_equilibrated(T.prepare_equilibrated(s, g), between=lambda: seen.append(1))
+    assert ok is True and seen == [1] and isinstance(d, np.ndarray)
+    assert rel(d, -(s * opo.apply(s * g, den))) <= 1e-12
+    bad = g.copy()
+    bad[5] = np.inf
+    tag, x = T.apply_equilibrated(T.prepare_equilibrated(s, bad))
+    assert tag == "x" and isinstance(x, np.ndarray) and x.shape == g.shape
 
 
 def test_vector_ops(gpu_device):

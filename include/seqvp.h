@@ -11,7 +11,8 @@
  *     k[38]=activation-delta;
  *   - return 0 on success, a negative SEQVP_E* code otherwise; nothing throws across the ABI;
  *   - all work is enqueued on `stream`; results are valid after that stream is synchronised.
- *   - the library holds no mutable global state: calls are reentrant across host threads/streams.
+ *   - the library holds no mutable global state, so calls are reentrant across host threads/streams; the one
+ *     exception is seqvp_q1_soft_energy_noalloc's fixed scratch array (see there).
  */
 #ifndef SEQVP_H
 #define SEQVP_H

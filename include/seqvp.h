@@ -33,7 +33,7 @@ const char* seqvp_version(void);
 
 /* Workspace bytes an entry point needs for a grid of nx*ny*nz nodes.
  * op: 0 soft energy, 1 barrier energy, 2 gradient, 3 GGN diagonal, 4 joint outer gradients,
- *     5 JD root, 6 DST apply, 7 dot products, 8 node-min JD. */
+ *     5 JD root, 6 DST apply, 7 dot products, 8 node-min JD, 10 trial energy (soft + barrier). */
 size_t seqvp_workspace_bytes(int op, int64_t nx, int64_t ny, int64_t nz);
 
 /* Soft JD/curl change energies (replaces FusedQ1.soft_energy,
@@ -44,6 +44,14 @@ size_t seqvp_workspace_bytes(int op, int64_t nx, int64_t ny, int64_t nz);
 int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base, const double* k,
                          int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                          size_t workspace_bytes, void* stream);
+
+/* The trial's soft and barrier energies in one sweep (replaces the pair FusedQ1.soft_energy + barrier_energy of
+ * AcceleratedHybridBackend._energy, accelerated.py:1574-1664): out[0:3] = seqvp_q1_soft_energy's [jd, curl,
+ * difference], out[3:7] = seqvp_q1_barrier_energy's [value, min J, unweighted change, min base J], bit for bit.
+ * `out` is 7 device doubles; workspace op 10 (12 doubles per 256-cell block). */
+int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
+                          int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
+                          size_t workspace_bytes, void* stream);
 
 /* seqvp_q1_soft_energy without a workspace and without any allocation. No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -197,6 +197,61 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
   block_reduce_store<3, 0>(s, none, partials + blk * 6);
 }
 
+// Closed27 barrier terms of one cell from its edges (and its base's): the minima of J (and base J) over the 27 points
+// into m, the barrier value into s[0] and, against a base, the unweighted stable change into s[1].
+template <bool WITH_BASE>
+__device__ __forceinline__ void barrier_cell(Edges& Eu, Edges& Eb, const double* __restrict__ k, DD s[2], double m[2]) {
+  const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+  // Minima over the 27 points, each with its compile-time closed blend, and whether any point is active.
+  bool active = false;
+  static_for<27>([&](auto P) {
+    if constexpr (P % 9 == 0) {
+      opaque_edges(Eu);
+      if (WITH_BASE) opaque_edges(Eb);
+    }
+    double V[3][3];
+    closed_point_grad<P>(Eu, V);
+    const double J = jacobian(V);
+    m[0] = nan_min(m[0], J);
+    if (WITH_BASE) {
+      double Y[3][3];
+      closed_point_grad<P>(Eb, Y);
+      const double Ja = jacobian(Y);
+      m[1] = nan_min(m[1], Ja);
+      active |= !(J >= act && Ja >= act);
+    } else {
+      active |= J < act;
+    }
+  });
+  // The barrier terms are zero at every point with J >= act (and, against a base, Ja >= act), so only a wave
+  // with an active point evaluates them, over the points in order with the tabulated blends.
+  if (__any(active)) {
+    double val = 0.0;
+    // points p = 3 row + ix in order; the three points of an x row share each field's x column
+#pragma unroll 1
+    for (int row = 0; row < 9; ++row) {
+      const double(&wr)[4] = CLOSED_W.w[3 * row][0];
+      double Ux[3], Bx[3];
+      column_w(Eu, 0, wr, Ux);
+      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+#pragma unroll 1
+      for (int p = 3 * row; p < 3 * row + 3; ++p) {
+        double V[3][3];
+        point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
+        const double J = jacobian(V);
+        val += barrier_value(J, w, delta, act, scale);
+        if (WITH_BASE) {
+          double Y[3][3];
+          point_grad_yz_w(Eb, CLOSED_W.w[p], Bx, Y);
+          const double Ja = jacobian(Y);
+          if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
+        }
+      }
+    }
+    dd_add(s[0], val);
+  }
+}
+
 // Closed27 barrier value, min J, unweighted stable change vs base, min base J
 // (residuals.py:173-186; energy_comparison.py:234-240; triton_q1_kernels barrier_energy).
 template <bool WITH_BASE>
@@ -211,7 +266,6 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
   if (c < d.ncell) {
     double ih[3];
     load_ih(k, ih);
-    const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
     const int64_t b = cell_base_node(c, d);
     Edges Eu, Eb;
     {
@@ -223,56 +277,87 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
         make_edges(v, ih, Eb);
       }
     }
-    // Minima over the 27 points, each with its compile-time closed blend, and whether any point is active.
-    bool active = false;
-    static_for<27>([&](auto P) {
-      if constexpr (P % 9 == 0) {
-        opaque_edges(Eu);
-        if (WITH_BASE) opaque_edges(Eb);
-      }
-      double V[3][3];
-      closed_point_grad<P>(Eu, V);
-      const double J = jacobian(V);
-      m[0] = nan_min(m[0], J);
-      if (WITH_BASE) {
-        double Y[3][3];
-        closed_point_grad<P>(Eb, Y);
-        const double Ja = jacobian(Y);
-        m[1] = nan_min(m[1], Ja);
-        active |= !(J >= act && Ja >= act);
-      } else {
-        active |= J < act;
-      }
-    });
-    // The barrier terms are zero at every point with J >= act (and, against a base, Ja >= act), so only a wave
-    // with an active point evaluates them, over the points in order with the tabulated blends.
-    if (__any(active)) {
-      double val = 0.0;
-      // points p = 3 row + ix in order; the three points of an x row share each field's x column
-#pragma unroll 1
-      for (int row = 0; row < 9; ++row) {
-        const double(&wr)[4] = CLOSED_W.w[3 * row][0];
-        double Ux[3], Bx[3];
-        column_w(Eu, 0, wr, Ux);
-        if (WITH_BASE) column_w(Eb, 0, wr, Bx);
-#pragma unroll 1
-        for (int p = 3 * row; p < 3 * row + 3; ++p) {
-          double V[3][3];
-          point_grad_yz_w(Eu, CLOSED_W.w[p], Ux, V);
-          const double J = jacobian(V);
-          val += barrier_value(J, w, delta, act, scale);
-          if (WITH_BASE) {
-            double Y[3][3];
-            point_grad_yz_w(Eb, CLOSED_W.w[p], Bx, Y);
-            const double Ja = jacobian(Y);
-            if (!(J >= act && Ja >= act)) dd_add(s[1], barrier_change(Ja, J, delta, act, scale));
-          }
-        }
-      }
-      dd_add(s[0], val);
-    }
+    barrier_cell<WITH_BASE>(Eu, Eb, k, s, m);
   }
   block_reduce_store<2, 2>(s, m, partials + blk * 6);
+}
+
+// The trial's soft and barrier energies in one sweep (the work of soft_energy_kernel and barrier_energy_kernel, in
+// the same per-cell order and the same blocks, so their partials are the same bits): the vertex loads and the edges
+// of the trial and its base are shared, and the closed-point pass runs on edges already in registers.
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __restrict__ u,
+                                                             const double* __restrict__ ref,
+                                                             const double* __restrict__ base,
+                                                             const double* __restrict__ k, Dims d,
+                                                             double* __restrict__ soft_partials,
+                                                             double* __restrict__ barrier_partials) {
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t c = blk * BLOCK + threadIdx.x;
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  DD sb[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  double m[2] = {__builtin_inf(), __builtin_inf()};
+  __shared__ double er_lds[36 * BLOCK];
+  lds_double* er = (lds_double*)(er_lds + threadIdx.x);
+  if (c < d.ncell) {
+    double ih[3];
+    load_ih(k, ih);
+    const double mu = k[K_MU], nu = k[K_NU];
+    const int64_t b = cell_base_node(c, d);
+    Edges Eu, Eb;
+    {
+      double v[8][3];
+      load_cell(u, b, d, v);
+      make_edges(v, ih, Eu);
+      load_cell(ref, b, d, v);
+      stage_edges(v, ih, er, BLOCK);
+      if (WITH_BASE) {
+        load_cell(base, b, d, v);
+        make_edges(v, ih, Eb);
+      }
+    }
+    double cj = 0.0, cc = 0.0;
+#pragma unroll 1
+    for (int row = 0; row < 9; ++row) {
+      const double(&wr)[4] = GAUSS_W.w[3 * row][0];
+      double Ux[3], Rx[3], Bx[3];
+      column_w(Eu, 0, wr, Ux);
+      column_col_w(er, BLOCK, 0, wr, Rx);
+      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+#pragma unroll 1
+      for (int p = 3 * row; p < 3 * row + 3; ++p) {
+        const double qw = k[K_QW + p];
+        double V[3][3], W[3][3], C[3], Cr[3];
+        point_grad_yz_w(Eu, GAUSS_W.w[p], Ux, V);
+        const double J = jacobian(V);
+        curl_of(V, C);
+        point_grad_col_yz_w(er, BLOCK, GAUSS_W.w[p], Rx, W);
+        const double Jr = jacobian(W);
+        curl_of(W, Cr);
+        const double dj = J - Jr;
+        const double d0 = C[0] - Cr[0], d1 = C[1] - Cr[1], d2 = C[2] - Cr[2];
+        cj += qw * dj * dj;
+        cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
+        if (WITH_BASE) {
+          double Y[3][3], Ca[3];
+          point_grad_yz_w(Eb, GAUSS_W.w[p], Bx, Y);
+          const double Ja = jacobian(Y);
+          curl_of(Y, Ca);
+          const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
+          double t = wm * ((J - Ja) * (dj + (Ja - Jr)));
+#pragma unroll
+          for (int i = 0; i < 3; ++i) t += wn * ((C[i] - Ca[i]) * ((C[i] - Cr[i]) + (Ca[i] - Cr[i])));
+          dd_add(s[2], t);
+        }
+      }
+    }
+    dd_add(s[0], 0.5 * mu * cj);
+    dd_add(s[1], 0.5 * nu * cc);
+    barrier_cell<WITH_BASE>(Eu, Eb, k, sb, m);
+  }
+  block_reduce_store<3, 0>(s, none, soft_partials + blk * 6);
+  block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1444,6 +1529,7 @@ static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
     case 5: return (size_t)nblocks * 3 * sizeof(double);
     case 8: return (size_t)d.ncell * sizeof(double);
     case 9: return (size_t)1024 * 8 * sizeof(double);
+    case 10: return (size_t)nblocks * 12 * sizeof(double);
     default: return 0;
   }
 }
@@ -1466,6 +1552,25 @@ extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const do
   else
     hipLaunchKernelGGL((soft_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
   hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
+                                     int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                     void* stream) {
+  if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(10, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  const Dims d = make_dims(nx, ny, nz);
+  const int64_t nb = cdiv(d.ncell, BLOCK);
+  hipStream_t s = (hipStream_t)stream;
+  double* soft = (double*)ws;
+  double* bar = soft + nb * 6;
+  if (base)
+    hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar);
+  else
+    hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, soft, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, bar, nb, out + 3);
   return launch_status();
 }
 

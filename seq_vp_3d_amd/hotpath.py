@@ -177,9 +177,9 @@ class PairState:
     def evaluate_energy(self, u, warped):
         q = self.q1
         node = q.node_energy(warped, self.fixed, u, self.ref, self.problem.weights.lam, data_weight=self.data_weight)
-        soft = q.soft_energy(u, self.ref)
-        bar = q.barrier_energy(u)
-        n, s, b = (x.tolist() for x in (node, soft, bar))
+        sb = q.trial_energy(u, self.ref)
+        n, v = node.tolist(), sb.tolist()
+        s, b = v[0:3], v[3:7]
         if not b[1] > q.delta:
             raise FloatingPointError("device barrier points contradict a feasible state")
         return Energies(n[0], n[1], s[0], s[1], b[0])
@@ -291,12 +291,10 @@ class PairState:
         node = q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
                              base_warped=self.warped, base_u=self.u)
         if screen:
-            soft = q.soft_energy_f32(self.u_t, self.ref, self.u)
-            bar = q.barrier_energy_f32(self.u_t, self.u)
+            sb = torch.cat((q.soft_energy_f32(self.u_t, self.ref, self.u), q.barrier_energy_f32(self.u_t, self.u)))
         else:
-            soft = q.soft_energy(self.u_t, self.ref, self.u)
-            bar = q.barrier_energy(self.u_t, self.u)
-        vals = torch.cat((node, soft, bar, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
+            sb = q.trial_energy(self.u_t, self.ref, self.u)       # soft and barrier terms in one sweep
+        vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
         n, s, b, nbad = vals[0:4], vals[4:7], vals[7:11], vals[11]
         if nbad:
             return TrialResult(False, "INVALID_DOMAIN", None)
@@ -371,9 +369,8 @@ class PairState:
         q, w = self.q1, self.problem.weights
         node = q.node_energy(self.warped, self.fixed, self.u, self.ref, w.lam, data_weight=self.data_weight,
                              base_warped=snap["warped"], base_u=snap["u"])
-        soft = q.soft_energy(self.u, self.ref, snap["u"])
-        bar = q.barrier_energy(self.u, snap["u"])
-        n, s_, b = node.tolist(), soft.tolist(), bar.tolist()
+        sb = q.trial_energy(self.u, self.ref, snap["u"]).tolist()
+        n, s_, b = node.tolist(), sb[0:3], sb[3:7]
         change = q.barrier_weight * b[2]
         return n[2] + n[3] + s_[2] + change, n[2] + change, min(b[1], b[3]) - q.delta
 

@@ -13,7 +13,7 @@ import torch
 from . import _lib
 from .problem import InputValidationError, problem_constants
 
-OP_SOFT, OP_BARRIER, OP_GRAD, OP_GGN, OP_JOINT, OP_ROOT, OP_DST, OP_DOTS, OP_NODEMIN, OP_NODE = range(10)
+OP_SOFT, OP_BARRIER, OP_GRAD, OP_GGN, OP_JOINT, OP_ROOT, OP_DST, OP_DOTS, OP_NODEMIN, OP_NODE, OP_TRIAL = range(11)
 
 
 class FusedQ1:
@@ -63,6 +63,16 @@ class FusedQ1:
         w, n = self._workspace(OP_BARRIER)
         _lib.call("seqvp_q1_barrier_energy", _lib.ptr(u), _lib.ptr(base), _lib.ptr(self.k), *self._dims(),
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        return out
+
+    def trial_energy(self, u, ref, base=None):
+        """soft_energy(u, ref, base) and barrier_energy(u, base) from one sweep: [jd, curl, difference, barrier
+        value, min J, unweighted barrier change, min base J], the same bits as the two calls."""
+        self._check(u, ref) if base is None else self._check(u, ref, base)
+        out = torch.empty(7, dtype=torch.float64, device=self.device)
+        w, n = self._workspace(OP_TRIAL)
+        _lib.call("seqvp_q1_trial_energy", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
+                  *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
     def soft_energy_f32(self, u, ref, base=None):

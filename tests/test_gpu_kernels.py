@@ -65,6 +65,10 @@ def test_soft_and_barrier_energy(gpu_device, activation):
     assert abs(w * bar[2] - chg) <= 1e-12 * max(abs(val), abs(chg), 1e-300)
     zero = q.soft_energy(R, R, R).cpu().numpy()
     assert np.array_equal(zero, np.zeros(3))
+    # the fused trial sweep: the same bits as the two separate sweeps, with and without a base
+    for base in (B, None):
+        fused = q.trial_energy(U, R, base)
+        assert torch.equal(fused, torch.cat((q.soft_energy(U, R, base), q.barrier_energy(U, base)))), base is None
 
 
 def test_energy_nan_propagates_to_min_j(gpu_device):

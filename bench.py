@@ -289,6 +289,18 @@ def serving_form(args, dev):
     mixed, wall, server = serve(reqs, offsets, dev, slots=args.slots)
     print(f"served {wall:.2f}s", file=sys.stderr, flush=True)
     same = all(a.theta_digest == b.theta_digest and a.energy == b.energy for a, b in zip(alone, mixed))
+    for a, b in zip(alone, mixed):
+        if a.theta_digest != b.theta_digest:
+            k = next((i for i, (x, y) in enumerate(zip(a.trajectory, b.trajectory)) if x != y),
+                     min(len(a.trajectory), len(b.trajectory)))
+            print(f"request {a.request_id} {a.shape}: attempts {len(a.trajectory)} alone / {len(b.trajectory)} "
+                  f"mixed, first difference at attempt {k}: alone {a.trajectory[k - 1:k + 2]} mixed "
+                  f"{b.trajectory[k - 1:k + 2]}", file=sys.stderr, flush=True)
+    # final-energy tolerance of the forms: each served result against the same request run alone on the plain path
+    ediff = max(abs(a.energy - b.energy) / max(abs(a.energy), 1e-300) for a, b in zip(alone, mixed))
+    statuses = {}
+    for r in mixed:
+        statuses[r.status] = statuses.get(r.status, 0) + 1
     recert = sum(r.recertified == "certified_feasible" for r in mixed)
     lat_mix, lat_alone = latency_summary(mixed), latency_summary(alone)
     n = len(reqs)
@@ -300,7 +312,11 @@ def serving_form(args, dev):
                             sizes={"x".join(map(str, k)): sum(1 for r in reqs if r.shape == k)
                                    for k in sorted(set(r.shape for r in reqs))},
                             outer_steps_per_request=args.outer, slots=args.slots,
-                            pair_seeds=f"{seeds[0]}..{seeds[-1]}"),
+                            pair_seeds=f"{seeds[0]}..{seeds[-1]}",
+                            result=("each request: the certified FP64 PG solve from u = 0 until its outer loop stops "
+                                    f"(convergence, or {args.outer} accepted outer steps), T66 root on; the output "
+                                    "map re-certified by the CPU interval certificate before it is returned")),
+                final_energy_max_rel_diff_vs_alone=ediff, final_status=statuses,
                 serial_pairs_per_hour=round(n / serial_s * 3600, 1), speedup_vs_serial=round(serial_s / wall, 3),
                 wall_seconds=round(wall, 3), serial_seconds=round(serial_s, 3),
                 latency_seconds=dict(mixed={k: round(v, 4) for k, v in lat_mix.items()},

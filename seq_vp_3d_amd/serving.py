@@ -72,6 +72,7 @@ class Result:
     started: float
     finished: float
     seconds: dict = field(default_factory=dict)
+    trajectory: list = field(default_factory=list)     # (reason, tau, trial energy) per line-search attempt
 
     @property
     def latency(self):
@@ -98,8 +99,10 @@ def run_request(req, device, stream=None, recert_workers=8):
         st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=device))
         t1 = time.perf_counter()
         secs["setup"] = t1 - started
+        trajectory = []
         res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=req.jd_step_global),
-                               max_outer=req.max_outer)
+                               max_outer=req.max_outer,
+                               inner_observer=lambda r: trajectory.append((r["reason"], r["tau"], r["energy_trial"])))
         t2 = time.perf_counter()
         secs["solve"] = t2 - t1
         theta = st.theta.cpu().numpy()
@@ -113,7 +116,7 @@ def run_request(req, device, stream=None, recert_workers=8):
         rec, low = cert.status, cert.min_jd_lower_bound
     secs["recertify"] = time.perf_counter() - t2
     return Result(req.request_id, req.shape, res.status, res.outer_accepted, energy, digest, rec, low, req.arrival,
-                  started, time.perf_counter(), secs)
+                  started, time.perf_counter(), secs, trajectory)
 
 
 class Server:

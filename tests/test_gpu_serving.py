@@ -50,3 +50,20 @@ def test_admission_respects_the_hbm_budget(gpu_device):
     with pytest.raises(MemoryError):
         s.submit(big).result()
     s.close()
+
+
+def test_native_pair_in_the_mix_is_the_same_alone(gpu_device):
+    """Uneven sizes in flight together: one 160x192x224 pair with 96^3 and 128^3 pairs on three slots. Every
+    result is re-certified by the CPU interval certificate and is bit for bit its alone result."""
+    from seq_vp_3d_amd.serving import serial, serve
+    shapes = [(160, 192, 224), (96, 96, 96), (128, 128, 128), (96, 96, 96)]
+    reqs = _requests(gpu_device, shapes, max_outer=2)
+    print("case serving native mix serial", flush=True)
+    alone, _ = serial(reqs, gpu_device)
+    print("case serving native mix served", flush=True)
+    mixed, _, server = serve(reqs, None, gpu_device, slots=3)
+    assert max(a[4] for a in server.admissions) >= 2
+    for a, b in zip(alone, mixed):
+        assert a.theta_digest == b.theta_digest and a.energy == b.energy and a.status == b.status, a.request_id
+        assert a.recertified == b.recertified == "certified_feasible"
+        assert a.outer_accepted == b.outer_accepted == 2

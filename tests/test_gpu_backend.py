@@ -133,3 +133,45 @@ def test_reference_flow_on_the_backend(gpu_device, jd_step_global, initial_step)
     used = {"evaluate_trial", "promote_trial", "gradient", "outer_norms", "poisson", "diagonal", "node_min_jd",
             "apply_equilibrated", "stable_energy_reduction"} | ({"global_jd_root"} if jd_step_global else set())
     assert used <= set(bp.calls), sorted(used - set(bp.calls))
+
+
+def _classify(actual, resolution):
+    """accept / within the resolution / increase, as the inner loop reads a reduction (energy_comparison.py:117-124)."""
+    return "accept" if actual - resolution >= 0 else "resolution" if abs(actual) <= resolution else "increase"
+
+
+def test_near_margin_decisions_match_the_longdouble_comparison(gpu_device):
+    """The reference sends a device energy difference within 8 x 4096 eps of the decision margin to its CPU long
+    double comparison (accelerated.py:3084-3111). The device path here decides every trial on its double-double
+    sums; on trials placed around both margins (actual = resolution and actual = 0) its reductions equal the
+    oracle's long double reductions of the same states to a thousandth of the resolution, and every decision is
+    the same."""
+    from backend_pair import BackendPair
+    from seq_vp_3d_amd.backend import AmdBackend
+    print("case near-margin reductions 14x15x16", flush=True)
+    spec, pair, w = _pair()
+    shape, spacing = spec.shape, (1., 1., 1.)
+    bp = BackendPair(AmdBackend(gpu_device), shape, spacing, w, pair.fixed.numpy(), pair.moving.numpy())
+    op = od.Pair(shape, spacing, w, pair.fixed.numpy(), pair.moving.numpy())
+    g = op.grid
+    theta = Q.restrict_free(.5 * pair.displacement.numpy(), g)
+    ref = Q.inject_free(.8 * theta, g)
+    base = bp.evaluate(theta, ref)
+    grad = bp.gradient(base, ref)
+    p = -grad / np.linalg.norm(grad)
+    slope = float(np.linalg.norm(grad))                    # actual(h) = h * slope + O(h^2) along p
+    h0 = 16 * od.EPS * od.total(base.energy) / slope       # the step where actual reaches the resolution
+    near = {"accept": 0, "resolution": 0, "increase": 0}
+    for f in (-4., -1., -.5, 1e-3, .25, .5, .9, .97, 1.03, 1.1, 2., 4., 16.):
+        trial = bp.evaluate(theta + f * h0 * p, ref)
+        assert isinstance(trial, od.State)
+        for outer_only in (False, True):
+            actual, resolution, certified = bp.reduction(base, trial, ref, outer_only=outer_only)
+            want = od.Pair.reduction(op, base, trial, ref, outer_only=outer_only)
+            assert resolution == want[1]
+            assert abs(actual - want[0]) <= 1e-3 * want[1], (f, outer_only, actual, want[0], want[1])
+            kind = _classify(want[0], want[1])
+            assert _classify(actual, resolution) == kind, (f, outer_only, actual, want)
+            if not outer_only and min(abs(want[0] - want[1]), abs(want[0])) <= 8 * want[1]:
+                near[kind] += 1
+    assert all(v >= 1 for v in near.values()), near

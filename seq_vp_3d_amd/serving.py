@@ -12,7 +12,6 @@ Every kernel is deterministic and a request's work is ordered on its own stream,
 same alone and in the mix (tests/test_gpu_serving.py checks it). The reference serves pairs one at a time from
 one process (scripts/run_stage_3d_e48_pg.py:874-918).
 """
-import hashlib
 import threading
 import time
 from concurrent.futures import Future
@@ -20,6 +19,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 import torch
+import xxhash
 
 from .certificate import CertificateConfig, certify_map
 from .hotpath import PairState
@@ -65,7 +65,7 @@ class Result:
     status: str
     outer_accepted: int
     energy: float
-    theta_digest: str
+    theta_digest: str                 # XXH3-128 of the output free vector's bytes (bitwise identity of results)
     recertified: str | None
     min_jd_lower_bound: float | None
     arrival: float
@@ -83,6 +83,12 @@ def _upload(a, device):
     if isinstance(a, torch.Tensor):
         return a.to(device=device, dtype=torch.float64, non_blocking=True).contiguous()
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device, non_blocking=True)
+
+
+def _download(t):
+    host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    host.copy_(t, non_blocking=True)
+    return host
 
 
 def run_request(req, device, stream=None, recert_workers=8):
@@ -105,10 +111,13 @@ def run_request(req, device, stream=None, recert_workers=8):
                                inner_observer=lambda r: trajectory.append((r["reason"], r["tau"], r["energy_trial"])))
         t2 = time.perf_counter()
         secs["solve"] = t2 - t1
-        theta = st.theta.cpu().numpy()
-        u = st.u.cpu().numpy() if req.recertify else None
+        # downloads into page-locked buffers (torch's host cache reuses them across requests), one wait
+        pinned = [_download(st.theta)] + ([_download(st.u)] if req.recertify else [])
+        torch.cuda.current_stream(device).synchronize()
+        theta = pinned[0].numpy()
+        u = pinned[1].numpy() if req.recertify else None
         energy = res.energy.total
-    digest = hashlib.sha256(theta.tobytes()).hexdigest()
+    digest = xxhash.xxh3_128_hexdigest(memoryview(theta).cast("B"))
     rec = None
     low = None
     if u is not None:

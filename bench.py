@@ -24,6 +24,7 @@ sys.path.insert(0, str(ROOT))
 FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 14445.,
     "seqvp_q1_barrier_energy": 8460.,
+    "seqvp_q1_trial_energy": 14445. + 8460.,
     "seqvp_q1_gradient": 12971. + 8093.,
     "seqvp_q1_gradient_free": 12971. + 8093.,
     "seqvp_q1_ggn_diagonal": 27290.,
@@ -37,6 +38,7 @@ FLOPS_PER_UNIT = {
 EXECUTED_FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 7223.,
     "seqvp_q1_barrier_energy": 1620.,
+    "seqvp_q1_trial_energy": 8699.,
     "seqvp_q1_gradient": 7542. + 24.,
     "seqvp_q1_gradient_free": 7542. + 24.,
     "seqvp_q1_ggn_diagonal": 12529.,
@@ -50,6 +52,7 @@ FP64_PEAK_TFLOPS = 78.6
 TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_soft_energy": 986e6,
     "seqvp_q1_barrier_energy": 362e6,
+    "seqvp_q1_trial_energy": 988e6,
     "seqvp_q1_gradient": 3666e6,
     "seqvp_q1_gradient_free": 3940e6,
     "seqvp_q1_joint_free": 5973e6,

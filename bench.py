@@ -46,6 +46,9 @@ EXECUTED_FLOPS_PER_UNIT = {
     "seqvp_spline_warp": 1015.,
 }
 FP64_PEAK_TFLOPS = 78.6
+# FP64 vector FMAs one MI355X sustains on a register-only stream (bench.py --ceilings, profiles/fp64_ceilings_mi355x.md;
+# about 2.25 GHz under load against the 2.4 GHz of the published peak)
+FP64_VALU_SUSTAINED_TFLOPS = 66.9
 # HBM bytes per launch at 160x192x224 from the FETCH_SIZE and WRITE_SIZE counter runs (rocprofv3 --pmc, one counter
 # per run, profiles/hbm_traffic_native_pmc.md), each counter scaled by its reading for a copy of known size
 # (--copy-probe: fetch x2.0, write x1.0)
@@ -80,6 +83,12 @@ def parse():
                     help="initial map: u = 0 (the configuration's start) or half the known displacement")
     ap.add_argument("--fp32-screen", action="store_true", help="precision form: FP32 screening energies")
     ap.add_argument("--outer", type=int, default=20, help="precision workload: outer steps of each FP64 solve")
+    ap.add_argument("--busy", nargs=2, metavar=("KERNEL_TRACE_CSV", "FORMS_LINE_FILE"),
+                    help="GPU busy share of the serial and served phases of a batch/serve run traced by rocprofv3 "
+                         "--kernel-trace; prints one JSON line and exits")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="measure the FP64 MFMA and FP64 VALU rates one GPU sustains on register-only instruction "
+                         "streams (seqvp_fp64_ceiling_probe), print one JSON line and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="before the workload, one seqvp_scale launch over 1 GiB (reads 1 GiB, writes 1 GiB): the "
                          "plain copy of known size that calibrates the FETCH_SIZE/WRITE_SIZE counters")
@@ -253,6 +262,57 @@ def file_form(args, pair, dev):
         shutil.rmtree(root, ignore_errors=True)
 
 
+def gpu_busy(trace_csv, line_file):
+    """GPU busy share of the serial and served phases of a forms run: the union of the kernel intervals of a
+    rocprofv3 kernel trace (CSV) inside each phase's CLOCK_MONOTONIC window from the run's JSON line."""
+    import csv
+    line = json.loads([ln for ln in open(line_file) if ln.startswith("{")][-1])
+    spans = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                   for r in csv.DictReader(open(trace_csv)))
+    out = {}
+    for phase, (t0, t1) in line["phase_monotonic_ns"].items():
+        sel = [(a, b) for a, b, _ in spans if a >= t0 and b <= t1]
+        busy, cur = 0, None
+        for a, b in sel:
+            if cur is None or a > cur[1]:
+                busy += 0 if cur is None else cur[1] - cur[0]
+                cur = [a, b]
+            else:
+                cur[1] = max(cur[1], b)
+        busy += 0 if cur is None else cur[1] - cur[0]
+        out[phase] = dict(wall_s=round((t1 - t0) / 1e9, 3), kernels=len(sel),
+                          kernel_time_s=round(sum(b - a for a, b in sel) / 1e9, 3),
+                          busy_s=round(busy / 1e9, 3), busy_share=round(busy / (t1 - t0), 3))
+    print(json.dumps(dict(metric="GPU busy share per phase", workload=line["config"]["workload"], **out)), flush=True)
+
+
+def fp64_ceilings(dev):
+    """Sustained FP64 rates on register-only streams, timed with HIP events on the launch stream: the best over
+    1-8 workgroups per CU (256 CUs) of v_mfma_f64_16x16x4f64 and of vector FMAs, against the published 78.6 TFLOP/s."""
+    import torch
+
+    from seq_vp_3d_amd import _lib
+    out = torch.empty(8 * 256 * 256, dtype=torch.float64, device=dev)
+    stream = _lib.stream_handle(dev)
+    rows = {}
+    for kind, name, iters, per in ((0, "mfma", 2000, lambda b, it: b * 4 * it * 8 * 2048.),
+                                   (1, "valu", 20000, lambda b, it: b * 256 * it * 8 * 2.)):
+        best = []
+        for per_cu in (1, 2, 4, 8):
+            blocks = 256 * per_cu
+            _lib.call("seqvp_fp64_ceiling_probe", kind, blocks, 10, _lib.ptr(out), stream)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            _lib.call("seqvp_fp64_ceiling_probe", kind, blocks, iters, _lib.ptr(out), stream)
+            b.record()
+            b.synchronize()
+            tf = per(blocks, iters) / (a.elapsed_time(b) * 1e-3) / 1e12
+            best.append((round(tf, 2), per_cu))
+        rows[name] = dict(tflops=max(best)[0], workgroups_per_cu=max(best)[1], sweep=best)
+    print(json.dumps(dict(metric="sustained FP64 rate on register-only streams", unit="TFLOP/s",
+                          published_peak=FP64_PEAK_TFLOPS, **rows)), flush=True)
+
+
 def serving_form(args, dev):
     """The batching (all requests at t = 0) and serving (Poisson arrivals) forms against serial one-pair-at-a-time
     execution of the same requests; each request runs `--outer` certified outer steps from u = 0 and its output is
@@ -286,10 +346,17 @@ def serving_form(args, dev):
         warm.setdefault(r.shape, Request(r.fixed, r.moving, max_outer=1, request_id=-1))
     serial(list(warm.values()), dev)
     torch.cuda.synchronize()
+    phases = {}                     # CLOCK_MONOTONIC ns, the clock of rocprofv3's kernel timestamps
+    t_a = time.monotonic_ns()
     alone, serial_s = serial(reqs, dev)
+    torch.cuda.synchronize()
+    phases["serial"] = [t_a, time.monotonic_ns()]
     print(f"serial {serial_s:.2f}s", file=sys.stderr, flush=True)
     offsets = None if args.workload == "batch" else poisson_offsets(len(reqs), args.rate, 7)
+    t_a = time.monotonic_ns()
     mixed, wall, server = serve(reqs, offsets, dev, slots=args.slots)
+    torch.cuda.synchronize()
+    phases["served"] = [t_a, time.monotonic_ns()]
     print(f"served {wall:.2f}s", file=sys.stderr, flush=True)
     same = all(a.theta_digest == b.theta_digest and a.energy == b.energy for a, b in zip(alone, mixed))
     for a, b in zip(alone, mixed):
@@ -325,7 +392,7 @@ def serving_form(args, dev):
                 latency_seconds=dict(mixed={k: round(v, 4) for k, v in lat_mix.items()},
                                      alone={k: round(v, 4) for k, v in lat_alone.items()}),
                 max_concurrent=max(a[4] for a in server.admissions), recertified=f"{recert}/{n}",
-                same_alone_and_in_mix=same,
+                same_alone_and_in_mix=same, phase_monotonic_ns=phases,
                 per_request=[dict(id=r.request_id, shape=list(r.shape), latency=round(r.latency, 4),
                                   solve=round(r.seconds["solve"], 4), setup=round(r.seconds["setup"], 4),
                                   recertify=round(r.seconds["recertify"], 4), status=r.status) for r in mixed])
@@ -338,6 +405,8 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (a pair is not sharded "
               f"across GPUs here; independent pairs would be placed per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.busy:
+        return gpu_busy(*args.busy)
     import numpy as np
     import torch
 
@@ -348,6 +417,8 @@ def main():
     from seq_vp_3d_amd.synthetic import CONFIG_NATIVE, PairSpec, device_min_jd, device_sampler, make_pair
 
     dev = torch.device("cuda:0")
+    if args.ceilings:
+        return fp64_ceilings(dev)
     if args.copy_probe:
         n = 1 << 27                                   # 2^27 doubles = 1 GiB
         a = torch.ones(n, dtype=torch.float64, device=dev)
@@ -432,6 +503,8 @@ def main():
                     trials=stats["trials"], accepted=stats["accepted"]),
         roofline=dict(bound="valu", achieved=round(executed, 2), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                       frac=round(executed / FP64_PEAK_TFLOPS, 4),
+                      sustained_valu_ceiling=FP64_VALU_SUSTAINED_TFLOPS,
+                      frac_of_sustained=round(executed / FP64_VALU_SUSTAINED_TFLOPS, 4),
                       traffic=TRAFFIC_BYTES_NATIVE.get(dominant) if shape == (160, 192, 224) else None,
                       kernel=dominant, mean_ms=round(mean_ms, 4),
                       reference_count_rate=round(reference_rate, 2),

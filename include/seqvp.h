@@ -451,6 +451,11 @@ int seqvp_taper(const double* u, int64_t nx, int64_t ny, int64_t nz, const doubl
                 const double* rz, double* out, void* stream);
 /* out = a u (n doubles). */
 int seqvp_scale(const double* u, double a, double* out, int64_t n, void* stream);
+/* Measurement probe with no counterpart in the reference (for the roofline of the FP64 kernels): `blocks`
+ * workgroups of 256 lanes run `iters` rounds of 8 independent v_mfma_f64_16x16x4f64 (kind 0) or 8 independent FP64
+ * vector FMAs per lane (kind 1) on registers only; out receives blocks * 256 doubles. The caller times the launch:
+ * kind 0 performs blocks * 4 * iters * 8 * 2048 flop, kind 1 blocks * 256 * iters * 8 * 2. */
+int seqvp_fp64_ceiling_probe(int kind, int64_t blocks, int64_t iters, double* out, void* stream);
 /* (nz, ny, nx, 3) mm field <-> (3, nz-2, ny-2, nx-2) voxel-unit interior (float32 when fp32 = 1); the reverse
  * writes zero faces. */
 int seqvp_field_to_interior(const double* u, int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,

@@ -83,6 +83,7 @@ SIGNATURES = {
     "seqvp_lerp_axis": [c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_i64, c_vp, c_vp, c_vp, c_vp],
     "seqvp_taper": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "seqvp_scale": [c_vp, c_dbl, c_vp, c_i64, c_vp],
+    "seqvp_fp64_ceiling_probe": [c_int, c_i64, c_i64, c_vp, c_vp],
     "seqvp_field_to_interior": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
     "seqvp_interior_to_field": [c_vp, c_i64, c_i64, c_i64, c_dbl, c_dbl, c_dbl, c_int, c_vp, c_vp],
     "seqvp_axis_transform": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_sz, c_vp],

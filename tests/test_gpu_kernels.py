@@ -430,3 +430,30 @@ def test_device_denominator_matches_host(gpu_device, metric):
     host = build_poisson_inverse(grid, a=.21, b=.037, metric=metric).denominator
     devp = DevicePoisson(SpectralMetric(grid, .21, .037, metric), gpu_device)
     np.testing.assert_allclose(devp._denominator.cpu().numpy(), host, rtol=2e-16, atol=0)
+
+
+def test_fp64_ceiling_probe(gpu_device):
+    """The roofline probe's streams compute what they claim: the VALU stream is the recurrence x <- x m + c per
+    accumulator, the MFMA stream is linear in the number of rounds; bad arguments are refused."""
+    from seq_vp_3d_amd import _lib
+    out = torch.empty(2 * 256, dtype=torch.float64, device=gpu_device)
+    s = _lib.stream_handle(gpu_device)
+    _lib.call("seqvp_fp64_ceiling_probe", 1, 2, 5, _lib.ptr(out), s)
+    got = out.cpu().numpy()
+    lanes = np.arange(256)
+    want = np.zeros(256)
+    for j in range(8):
+        x = 1.0 + (lanes + j) * 1e-9
+        for _ in range(5):
+            x = x * 0.999999999 + 1e-9
+        want += x
+    np.testing.assert_allclose(got[:256], want, rtol=1e-14)
+    np.testing.assert_array_equal(got[256:], got[:256])
+    r = []
+    for iters in (3, 6):
+        _lib.call("seqvp_fp64_ceiling_probe", 0, 2, iters, _lib.ptr(out), s)
+        r.append(out.cpu().numpy().copy())
+    assert np.all(np.isfinite(r[0])) and np.all(r[0] > 0)
+    np.testing.assert_allclose(r[1], 2 * r[0], rtol=1e-13)
+    assert _lib.raw("seqvp_fp64_ceiling_probe", 2, 2, 5, _lib.ptr(out), s) != 0
+    assert _lib.raw("seqvp_fp64_ceiling_probe", 0, 0, 5, _lib.ptr(out), s) != 0

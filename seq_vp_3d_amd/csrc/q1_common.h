@@ -366,16 +366,18 @@ __device__ __forceinline__ double cofactor_jacobian(const double V[3][3], double
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int j = 0; j < 3; ++j) F[r][j] = V[r][j] + (r == j ? 1.0 : 0.0);
-  A[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
-  A[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
-  A[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+  // row 0 and J with jacobian()'s explicit fused multiply-adds, so J is its bits: a JD residual J - J_ref is
+  // exactly 0 where u = ref, whichever of the two computed each side
+  A[0][0] = fma(F[1][1], F[2][2], -(V[1][2] * V[2][1]));
+  A[0][1] = fma(V[1][2], V[2][0], -(V[1][0] * F[2][2]));
+  A[0][2] = fma(V[1][0], V[2][1], -(F[1][1] * V[2][0]));
   A[1][0] = F[2][1] * F[0][2] - F[2][2] * F[0][1];
   A[1][1] = F[2][2] * F[0][0] - F[2][0] * F[0][2];
   A[1][2] = F[2][0] * F[0][1] - F[2][1] * F[0][0];
   A[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
   A[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
   A[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
-  return (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
+  return fma(V[0][2], A[0][2], fma(F[0][0], A[0][0], V[0][1] * A[0][1]));
 }
 
 template <typename T>

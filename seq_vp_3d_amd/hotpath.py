@@ -116,6 +116,9 @@ class PairState:
         # synchronisation
         self.scalars = z(12)
         self._prev = None           # (theta, g, p) of the last accepted state, kept for the next BB2 products
+        # The accepted state's version (bumped whenever it changes), the version at which the reference was set to
+        # it, and the version whose outer gradient (image + barrier) the joint pass left in diag.
+        self._version, self._ref_at, self._gf_at = 0, None, None
         self.dot_ws = torch.empty(max(_lib.workspace_bytes(7, self.shape), 256), dtype=torch.uint8, device=self.device)
         config = certificate if certificate is not None else CertificateConfig(delta=problem.delta)
         if config.delta != problem.delta:
@@ -157,8 +160,10 @@ class PairState:
         """Set the accepted map from theta (and optionally the reference) and evaluate it. `certificate` is the
         map's certificate when the caller has one (else state_certificate() computes it on demand)."""
         self.certificate = certificate
+        self._version += 1
         if ref is not None:
             self.ref.copy_(torch.as_tensor(ref, dtype=torch.float64))
+            self._ref_at = None
         if theta is not None:
             self.theta.copy_(torch.as_tensor(theta, dtype=torch.float64))
         self.inject(self.theta, self.u)
@@ -192,6 +197,7 @@ class PairState:
         q.node_terms(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
                      ggn=self.node_a)
         diag_nodes = q.diagonal_nodes(self.node_a, self.u)
+        self._gf_at = None
         self.restrict(diag_nodes, self.diag)
         # alpha = factor * mean over free nodes of (q/omega) |b|^2 / 3 (q/omega = 1 on free nodes)
         self.restrict(self.bgrad, self.p)          # p is free until direction(); g holds the gradient
@@ -232,7 +238,18 @@ class PairState:
 
     # ---- gradient / direction / bound ------------------------------------------------------------
     def gradient(self):
-        """Full energy gradient on theta (device), g = restrict(node terms + soft + barrier VJPs)."""
+        """Full energy gradient on theta (device), g = restrict(node terms + soft + barrier VJPs).
+
+        Right after a self-reference refresh (ref = u) the proximity term lam q (u - ref) and every JD/curl change
+        residual J - J_ref, c - c_ref are exactly zero, so the total gradient is the outer gradient (image +
+        barrier) that outer_norms computed for the same state: that vector is taken instead of a second sweep (the
+        first gradient of each inner solve of the outer loop). J and J_ref come out of the same fused
+        multiply-adds, so the sweep's soft contributions are exact zeros there and both passes add the same barrier
+        contributions in the same order to the same image term: the vectors are the same bits."""
+        if self._gf_at == self._version and self._ref_at == self._version:
+            self.g, self.diag = self.diag, self.g
+            self._gf_at = None
+            return self.g
         q, w = self.q1, self.problem.weights
         return q.gradient_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam,
                                data_weight=self.data_weight, out=self.g)
@@ -338,6 +355,7 @@ class PairState:
                                   p=torch.empty_like(self.p))
             self.g, self._prev["g"] = self._prev["g"], self.g
             self.p, self._prev["p"] = self._prev["p"], self.p
+        self._version += 1
         self.u, self.u_t = self.u_t, self.u
         self.theta, self.theta_t = self.theta_t, self.theta
         self.warped, self.warped_t = self.warped_t, self.warped
@@ -356,6 +374,7 @@ class PairState:
                     energy=self.energy, certificate=self.certificate)
 
     def restore(self, snap):
+        self._version += 1
         self.theta.copy_(snap["theta"])
         self.u.copy_(snap["u"])
         self.warped.copy_(snap["warped"])
@@ -381,6 +400,7 @@ class PairState:
         # p, diag and g are free here (recomputed before use)
         gd, gf, ge = q.joint_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam,
                                   data_weight=self.data_weight, out=(self.p, self.diag, self.g))
+        self._gf_at = self._version
         ff, dd, ee = self.dots((gf, gf), (gd, gd), (ge, ge))
         r = float(np.sqrt(self.grid.omega))
         return float(np.sqrt(ff)) / r, float(np.sqrt(dd)) / r, float(np.sqrt(ee)) / r
@@ -389,6 +409,7 @@ class PairState:
         """Self reference: u_ref <- u (problem.py:397-403). The proximity, JD and curl change terms of the
         state are then exactly zero; image and barrier do not depend on the reference."""
         self.ref.copy_(self.u)
+        self._ref_at = self._version
         if self.energy is None:
             self.energy = self.evaluate_energy(self.u, self.warped)
         self.energy = Energies(self.energy.image, 0., 0., 0., self.energy.barrier)

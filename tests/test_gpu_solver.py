@@ -154,3 +154,84 @@ def test_root_launch_then_witness_cubics_then_read(gpu_device):
         pending = st.jd_bound_launch(limit)
         mem.predict(st.witness_coefficients, limit, .9)
         assert st.jd_bound_read(pending) == want, limit
+
+
+def _decode(v):
+    return float.fromhex(v["hex"]) if isinstance(v, dict) else v
+
+
+def test_device_solver_follows_the_oracle_trajectory_for_50_outer_steps(gpu_device):
+    """The 32^3 CPU-sized pair (12 blobs, true JD >= 0.6), 50 certified outer steps from u = 0 with the reference's
+    defaults: the device driver on PairState against the oracle driver's trajectory stored by
+    tests/golden/make_driver_trajectory.py (every attempt's decision, tau and trial energy and every outer row to
+    1e-12 relative; 97 entries of the final theta)."""
+    import json
+    from pathlib import Path
+    from types import SimpleNamespace
+
+    from test_driver_oracle import compare_runs
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+    from seq_vp_3d_amd.synthetic import CONFIG_SMALL, make_pair
+
+    path = Path(__file__).resolve().parent / "golden" / "driver_trajectory_32.json"
+    doc = json.loads(path.read_text())
+    print("case solver vs stored oracle trajectory 32^3,", doc["config"]["outer"], "outer steps", flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    pair = make_pair(CONFIG_SMALL, "cpu", sampler)
+    assert float(pair.fixed.sum()) == _decode(doc["pair"]["fixed_sum"])
+    assert float(pair.moving.sum()) == _decode(doc["pair"]["moving_sum"])
+    want = SimpleNamespace(status=doc["status"],
+                           history=[{k: _decode(v) for k, v in r.items()} for r in doc["history"]])
+    want_inner = [{k: _decode(v) for k, v in r.items()} for r in doc["inner_history"]]
+    w = native_weights(CONFIG_SMALL.shape)
+    gpu = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+    gpu.set_state(theta=torch.zeros(gpu.nfree, dtype=torch.float64, device=gpu_device))
+    rows, inner_rows = [], []
+    res = solve_sequential(gpu, SequentialConfig(), PGConfig(), max_outer=doc["config"]["outer"],
+                           observer=rows.append, inner_observer=inner_rows.append)
+    compare_runs(want, want_inner, res, rows, inner_rows)
+    theta = gpu.theta.cpu().numpy()
+    got = theta[np.asarray(doc["theta"]["index"])]
+    ref = np.array([_decode(v) for v in doc["theta"]["values"]])
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * scale)
+    assert abs(np.linalg.norm(theta) - _decode(doc["theta"]["norm"])) <= 1e-10 * _decode(doc["theta"]["norm"])
+
+
+def test_first_inner_gradient_after_refresh_is_the_outer_gradient(gpu_device):
+    """After a self-reference refresh the total gradient equals the outer (image + barrier) gradient of the joint
+    pass bit for bit, so PairState.gradient takes that vector; after any state change it sweeps again."""
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print("case gradient reuse after refresh 18x19x20", flush=True)
+    spec = PairSpec((18, 19, 20), 8, (2., 4.), 3, .6, .01, 5)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    st = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+    theta = Q.restrict_free(.7 * pair.displacement.numpy(), Q.Grid(spec.shape, (1., 1., 1.)))
+    st.set_state(theta=torch.tensor(theta, device=gpu_device))
+    st.refresh_reference()
+    st.outer_norms()
+    gf = st.diag.clone()
+    swept = st.q1.gradient_free(st.warped, st.fixed, st.bgrad, st.u, st.ref, w.lam, data_weight=st.data_weight).clone()
+    g = st.gradient()
+    assert torch.equal(g, gf) and torch.equal(g, swept)
+    assert float(torch.linalg.vector_norm(g)) > 0
+    # a new state (and then a stale joint vector) is swept again
+    st.set_state(theta=torch.tensor(.9 * theta, device=gpu_device))
+    g2 = st.gradient().clone()
+    assert not torch.equal(g2, gf)
+    want = st.q1.gradient_free(st.warped, st.fixed, st.bgrad, st.u, st.ref, w.lam, data_weight=st.data_weight)
+    assert torch.equal(g2, want)

@@ -262,6 +262,12 @@ int seqvp_restrict(const double* u, int64_t nx, int64_t ny, int64_t nz, double* 
 int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t nx, int64_t ny, int64_t nz,
                       double* theta_out, double* u, void* stream);
 
+/* The same candidate map with the step read from device memory (tau: one double), so a captured trial segment
+ * (seqvp_graph_*) takes a new step on every launch; same arithmetic, same bits. Counterpart of the host axpy of
+ * poisson_gradient.py:1403-1437 inside the device trial of accelerated.py:1666-1730. */
+int seqvp_axpy_inject_dev(const double* theta, const double* p, const double* tau, int64_t nx, int64_t ny, int64_t nz,
+                          double* theta_out, double* u, void* stream);
+
 /* BB2 secant products (replaces the host products of _step_proposal, poisson_gradient.py:955-990):
  * with s = theta - theta_old, y = g - g_old, iy = p_old - p (each rounded once), out[0] = s.y,
  * out[1] = s.s, out[2] = y.y, out[3] = y.iy, compensated (double-double) sums. */
@@ -462,6 +468,19 @@ int seqvp_field_to_interior(const double* u, int64_t nx, int64_t ny, int64_t nz,
                             int fp32, void* out, void* stream);
 int seqvp_interior_to_field(const void* v, int64_t nx, int64_t ny, int64_t nz, double hx, double hy, double hz,
                             int fp32, double* out, void* stream);
+
+/* HIP graphs of device-only solver segments (replaces the per-attempt host round trips of the reference's device
+ * trial and direction, accelerated.py:1666-1730, hybrid.py:262-286, poisson_gradient.py:1241-1262, with one graph
+ * launch). seqvp_graph_begin starts a thread-local capture on `stream` (not the null stream); the library calls
+ * that follow on that stream are recorded, not run; seqvp_graph_end closes the capture and instantiates it into
+ * *exec (SEQVP_ELAUNCH if a recorded call failed). seqvp_graph_launch runs it on any stream; seqvp_graph_destroy
+ * frees it. A graph holds the device addresses it was captured with: the caller keys graphs by them. */
+int seqvp_graph_begin(void* stream);
+int seqvp_graph_end(void* stream, void** exec);
+int seqvp_graph_launch(void* exec, void* stream);
+int seqvp_graph_destroy(void* exec);
+/* bytes from src to dst (device or page-locked host memory) on `stream`; recorded as a copy node under capture. */
+int seqvp_copy_async(void* dst, const void* src, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

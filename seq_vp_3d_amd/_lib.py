@@ -57,6 +57,12 @@ SIGNATURES = {
     "seqvp_inject": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_axpy_inject_dev": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_graph_begin": [c_vp],
+    "seqvp_graph_end": [c_vp, ctypes.POINTER(c_vp)],
+    "seqvp_graph_launch": [c_vp, c_vp],
+    "seqvp_graph_destroy": [c_vp],
+    "seqvp_copy_async": [c_vp, c_vp, c_sz, c_vp],
     "seqvp_bb2_products": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_step_bound": [c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz,
                          c_vp],
@@ -154,6 +160,8 @@ TIMER = None
 
 def call(name, *args):
     timer = TIMER
+    if timer is not None and torch.cuda.is_current_stream_capturing():
+        timer = None                # a captured call runs later, inside its graph: nothing to bracket now
     if timer is not None:
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record()

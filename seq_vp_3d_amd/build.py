@@ -18,7 +18,7 @@ LIB = HERE / "libseqvp_hip.so"
 OBJ = HERE / "_build"
 
 SOURCES = ("q1_sweeps.hip", "prescreen.hip", "certificate.hip", "spline.hip", "dst.hip", "vec.hip", "preopt.hip",
-           "resample.hip", "ceiling.hip", "abi.hip")
+           "resample.hip", "ceiling.hip", "graph.hip", "abi.hip")
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 PER_FILE = {"prescreen.hip": ["-ffp-contract=off"], "certificate.hip": ["-ffp-contract=off"],

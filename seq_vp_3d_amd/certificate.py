@@ -201,15 +201,17 @@ class DeviceCertificate:
         self.codes = torch.empty(grid.n_cells, dtype=torch.uint8, device=self.device) if keep_codes else None
         self._u = None
 
-    def screen(self, u):
-        """Enqueue the screen of u (nz, ny, nx, 3) float64 on the current stream."""
+    def screen(self, u, summary=None):
+        """Enqueue the screen of u (nz, ny, nx, 3) float64 on the current stream. `summary`: 5 device doubles that
+        receive the screen values instead of self.summary (the caller then passes their host copy to finish())."""
         if u.dtype != torch.float64 or tuple(u.shape) != tuple(self.grid.shape) + (3,) or not u.is_contiguous():
             raise InputValidationError("u must be a contiguous (nz, ny, nx, 3) float64 device tensor")
         nz, ny, nx = self.grid.shape
         hx, hy, hz = (float(h) for h in self.grid.spacing)
         _lib.call("seqvp_certificate_screen", _lib.ptr(u), _lib.ptr(self.D), hx, hy, hz, self.config.delta, nx, ny,
                   nz, int(self.config.tile_size), _lib.ptr(self.codes), _lib.ptr(self.undecided), self.capacity,
-                  _lib.ptr(self.summary), _lib.ptr(self.ws), self.ws_bytes, _lib.stream_handle(self.device))
+                  _lib.ptr(self.summary if summary is None else summary), _lib.ptr(self.ws), self.ws_bytes,
+                  _lib.stream_handle(self.device))
         self._u = u
 
     def finish(self, summary=None):

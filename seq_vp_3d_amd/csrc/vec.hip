@@ -87,11 +87,14 @@ __global__ void __launch_bounds__(VB) restrict_kernel(const double* __restrict__
 }
 
 // u_trial = inject(theta + tau p): the candidate map of the line search (poisson_gradient.py:1403-1437)
-// written straight into the node layout, with the candidate theta itself.
+// written straight into the node layout, with the candidate theta itself. tau_dev (when not NULL) holds the step in
+// device memory, for captured segments whose step changes between launches.
 __global__ void __launch_bounds__(VB) axpy_inject_kernel(const double* __restrict__ theta, const double* __restrict__ p,
-                                                         double tau, int64_t nx, int64_t ny, int64_t nz,
+                                                         double tau_arg, const double* __restrict__ tau_dev,
+                                                         int64_t nx, int64_t ny, int64_t nz,
                                                          double* __restrict__ theta_out, double* __restrict__ u) {
 #pragma clang fp contract(off)
+  const double tau = tau_dev ? *tau_dev : tau_arg;
   const int64_t nn = nx * ny * nz;
   const int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x;
   if (i >= nn) return;
@@ -317,8 +320,17 @@ extern "C" int seqvp_axpy_inject(const double* theta, const double* p, double ta
                                  double* theta_out, double* u, void* stream) {
   if (!theta || !p || !u || nx < 3 || ny < 3 || nz < 3) return SEQVP_EINVAL;
   const int64_t nn = nx * ny * nz;
-  hipLaunchKernelGGL(axpy_inject_kernel, dim3((nn + VB - 1) / VB), dim3(VB), 0, (hipStream_t)stream, theta, p, tau, nx,
-                     ny, nz, theta_out, u);
+  hipLaunchKernelGGL(axpy_inject_kernel, dim3((nn + VB - 1) / VB), dim3(VB), 0, (hipStream_t)stream, theta, p, tau,
+                     (const double*)nullptr, nx, ny, nz, theta_out, u);
+  return launch_status();
+}
+
+extern "C" int seqvp_axpy_inject_dev(const double* theta, const double* p, const double* tau, int64_t nx, int64_t ny,
+                                     int64_t nz, double* theta_out, double* u, void* stream) {
+  if (!theta || !p || !tau || !u || nx < 3 || ny < 3 || nz < 3) return SEQVP_EINVAL;
+  const int64_t nn = nx * ny * nz;
+  hipLaunchKernelGGL(axpy_inject_kernel, dim3((nn + VB - 1) / VB), dim3(VB), 0, (hipStream_t)stream, theta, p, 0.0, tau,
+                     nx, ny, nz, theta_out, u);
   return launch_status();
 }
 

@@ -28,8 +28,9 @@ import torch
 from . import _lib
 from .poisson import DevicePoisson, SpectralMetric
 from .certificate import CertificateConfig, DeviceCertificate
+from .graphs import SegmentGraphs, key_of
 from .jd_step import nominate_witness
-from .q1 import FusedJDRoot, FusedQ1
+from .q1 import OP_TRIAL, FusedJDRoot, FusedQ1
 from .spline import DeviceSpline
 
 JD_AFFINE_FLOOR = 1e-12
@@ -73,9 +74,14 @@ class TrialResult:
 
 
 class PairState:
-    """Device buffers of one pair on one grid (node fields (nz, ny, nx, 3), free vectors (3F,))."""
+    """Device buffers of one pair on one grid (node fields (nz, ny, nx, 3), free vectors (3F,)).
 
-    def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None, certificate=None):
+    With `graphs` (the default) the trial and the direction of each inner iteration run as captured HIP graphs
+    (seq_vp_3d_amd.graphs): the same library calls on the same buffers, launched as one graph each; without it they
+    are issued one by one. Both end with one copy of the segment's scalars to page-locked host memory and one wait."""
+
+    def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None, certificate=None,
+                 graphs=True):
         self.problem = problem
         self.grid = problem.grid
         self.device = torch.device(device)
@@ -132,6 +138,32 @@ class PairState:
         self.energy = None          # Energies of the accepted state
         self.metric_info = {}
         self._ext_stats = None
+        # Segment buffers. Trial: node energies (4) | soft + barrier (7) | invalid-sample count (int64 bits) |
+        # certificate screen (5); the step comes from a page-locked word through the segment's first copy.
+        self._tau_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
+        self._tau_dev = z(1)
+        self._trial_dev = z(17)
+        self._trial_host = torch.zeros(17, dtype=torch.float64, pin_memory=True)
+        self._scalars_host = torch.zeros(12, dtype=torch.float64, pin_memory=True)
+        self._tau_np, self._trial_np, self._scalars_np = (t.numpy() for t in (self._tau_host, self._trial_host,
+                                                                                 self._scalars_host))
+        # the shared workspace at its largest size from the start, so that no call reallocates it under a captured
+        # graph's addresses
+        sizes = [_lib.workspace_bytes(op, self.shape) for op in range(OP_TRIAL + 1)]
+        nz, ny, nx = self.shape
+        sizes.append(_lib.raw("seqvp_node_min_jd_extended_workspace_bytes", nx, ny, nz))
+        self.q1.ws.get(max(sizes))
+        self.graphs = SegmentGraphs(self.device) if graphs else None
+
+    def _segment(self, key, body):
+        """Runs a device-only segment: its captured graph for `key`, or its calls one by one without graphs."""
+        if self.graphs is None:
+            body()
+        else:
+            self.graphs.run(key, body)
+
+    def _wait(self):
+        torch.cuda.current_stream(self.device).synchronize()
 
     # ---- small helpers -------------------------------------------------------------------------
     def _stream(self):
@@ -246,13 +278,22 @@ class PairState:
         first gradient of each inner solve of the outer loop). J and J_ref come out of the same fused
         multiply-adds, so the sweep's soft contributions are exact zeros there and both passes add the same barrier
         contributions in the same order to the same image term: the vectors are the same bits."""
+        if not self._take_outer_gradient():
+            self._gradient_sweep()
+        return self.g
+
+    def _take_outer_gradient(self):
+        """Takes the joint pass's outer gradient as g when it is the total gradient of the current state."""
         if self._gf_at == self._version and self._ref_at == self._version:
             self.g, self.diag = self.diag, self.g
             self._gf_at = None
-            return self.g
+            return True
+        return False
+
+    def _gradient_sweep(self):
         q, w = self.q1, self.problem.weights
-        return q.gradient_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam,
-                               data_weight=self.data_weight, out=self.g)
+        q.gradient_free(self.warped, self.fixed, self.bgrad, self.u, self.ref, w.lam, data_weight=self.data_weight,
+                        out=self.g)
 
     def direction(self, fp32=False):
         """p = -S M^-1 (S g)."""
@@ -290,20 +331,44 @@ class PairState:
         """Candidate theta + tau p: certificate screen, sampling and energies against the accepted state.
         screen=True evaluates the soft and barrier terms with the FP32 screening kernels and returns the
         screening band of the differences (accelerated.py:1603-1612, 67-76)."""
-        nz, ny, nx = self.shape
-        q = self.q1
-        _lib.call("seqvp_axpy_inject", _lib.ptr(self.theta), _lib.ptr(self.p), float(tau), nx, ny, nz,
-                  _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
-        self.cert.screen(self.u_t)
         self._certificate = None
-        # value and gradient at once: the 64-tap gather dominates the warp, and an accepted trial then needs no
-        # second warp at promotion
-        _, _, bad = self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t)
-        self._trial_bad = bad
-        return self.trial_energies(screen)
+        if screen:
+            nz, ny, nx = self.shape
+            _lib.call("seqvp_axpy_inject", _lib.ptr(self.theta), _lib.ptr(self.p), float(tau), nx, ny, nz,
+                      _lib.ptr(self.theta_t), _lib.ptr(self.u_t), self._stream())
+            self.cert.screen(self.u_t)
+            _, _, self._trial_bad = self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t)
+            return self.trial_energies(screen)
+        # FP64 trial as one segment: the step through the page-locked word, the candidate, the certificate screen,
+        # the warp (value and gradient at once: the 64-tap gather dominates the warp, and an accepted trial then
+        # needs no second warp at promotion), the energies, and one copy of the 17 scalars to the host
+        self._tau_np[0] = tau
+        key = key_of("trial", self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t, self.warped_t,
+                     self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided, self.cert.capacity)
+        self._segment(key, self._trial_segment)
+        self.cert._u = self.u_t
+        self._wait()
+        vals = self._trial_np.tolist()
+        nbad = int(self._trial_np[11:12].view(np.int64)[0])
+        return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
+
+    def _trial_segment(self):
+        nz, ny, nx = self.shape
+        q, w, s = self.q1, self.problem.weights, self._stream()
+        d = self._trial_dev
+        _lib.call("seqvp_copy_async", _lib.ptr(self._tau_dev), ctypes.c_void_p(self._tau_host.data_ptr()), 8, s)
+        _lib.call("seqvp_axpy_inject_dev", _lib.ptr(self.theta), _lib.ptr(self.p), _lib.ptr(self._tau_dev), nx, ny, nz,
+                  _lib.ptr(self.theta_t), _lib.ptr(self.u_t), s)
+        self.cert.screen(self.u_t, summary=d[12:17])
+        self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t, status=d[11:12].view(torch.int64))
+        q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
+                      base_warped=self.warped, base_u=self.u, out=d[0:4])
+        q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11])      # soft and barrier terms in one sweep
+        _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
 
     def trial_energies(self, screen=False):
-        """Energies and stable differences of the current candidate (u_t, warped_t) against the accepted state."""
+        """Energies and stable differences of the current candidate (u_t, warped_t) against the accepted state,
+        after trial(tau, screen=True) (its sampling count and certificate screen)."""
         q, w = self.q1, self.problem.weights
         node = q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
                              base_warped=self.warped, base_u=self.u)
@@ -312,7 +377,12 @@ class PairState:
         else:
             sb = q.trial_energy(self.u_t, self.ref, self.u)       # soft and barrier terms in one sweep
         vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
-        n, s, b, nbad = vals[0:4], vals[4:7], vals[7:11], vals[11]
+        return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)
+
+    def _trial_result(self, n, s, b, nbad, summary, screen):
+        """The TrialResult of a candidate from its node energies, soft and barrier terms, invalid-sample count and
+        certificate screen values."""
+        q = self.q1
         if nbad:
             return TrialResult(False, "INVALID_DOMAIN", None)
         folded = not b[1] > q.delta or not np.isfinite(b[1])
@@ -323,7 +393,7 @@ class PairState:
             witness = nominate_witness(self.u_t, q.k, self.grid, q.delta, self.device)
             if witness is not None:
                 return TrialResult(False, "JACOBIAN_VIOLATION", None, witness=witness)
-        cert = self._certificate if self._certificate is not None else self.cert.finish(summary=vals[12:17])
+        cert = self._certificate if self._certificate is not None else self.cert.finish(summary=summary)
         self._certificate = cert
         if not cert.certified:
             reason = "FEASIBILITY_UNCERTAIN" if cert.status == "uncertain" else cert.reason
@@ -427,10 +497,29 @@ class PairState:
         sy, ss, yy2, yiy = out.tolist()
         return sy, float(np.sqrt(ss)), float(np.sqrt(yy2)), yiy
 
-    def direction_scalars(self, prev, norm=False, ratio=False):
+    def direction_scalars(self, prev, norm=False, ratio=False, fp32=None, gradient=False):
         """(g.p, BB2 products for `prev` or None, domain step bound, gradient norm or None, ratio of the last
         accepted step or None) with one host synchronisation: the reductions are enqueued back to back into
-        separate slots of one small buffer. `ratio` reads the slot step_ratio_launch filled."""
+        separate slots of one small buffer. `ratio` reads the slot step_ratio_launch filled. With `fp32` not None
+        the direction p = -S M^-1 (S g) (FP32 transforms when true) is computed first, and with `gradient` the
+        gradient before it (gradient()); all of it runs as one segment."""
+        sweep = gradient and not self._take_outer_gradient()
+        prev_key = None if prev is None else (prev["theta"], prev["g"], prev["p"])
+        key = key_of("direction", sweep, fp32, norm, *(prev_key or (None,)), self.g, self.p, self.u, self.theta,
+                     self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale, self._den,
+                     self.q1.ws.buf, self.dot_ws, self.scalars)
+        self._segment(key, lambda: self._direction_segment(prev, norm, fp32, sweep))
+        self._wait()
+        v = self._scalars_np.tolist()
+        bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
+        G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None
+        return v[0], bb2, v[8], G, (self._ratio(v[9], v[10]) if ratio else None)
+
+    def _direction_segment(self, prev, norm, fp32, sweep):
+        if sweep:
+            self._gradient_sweep()
+        if fp32 is not None:
+            self.direction(fp32=fp32)
         out = self.scalars
         g, p = self.g, self.p
         xs, ys = (ctypes.c_void_p * 1)(g.data_ptr()), (ctypes.c_void_p * 1)(p.data_ptr())
@@ -452,10 +541,8 @@ class PairState:
         ox, oy, oz = self.grid.origin
         _lib.call("seqvp_step_bound", _lib.ptr(self.p), _lib.ptr(self.u), ox, oy, oz, hx, hy, hz, nx, ny, nz,
                   _lib.ptr(out[8:9]), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
-        v = out.tolist()
-        bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
-        G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None
-        return v[0], bb2, v[8], G, (self._ratio(v[9], v[10]) if ratio else None)
+        _lib.call("seqvp_copy_async", ctypes.c_void_p(self._scalars_host.data_ptr()), _lib.ptr(out), 12 * 8,
+                  self._stream())
 
     def step_bound(self):
         nz, ny, nx = self.shape

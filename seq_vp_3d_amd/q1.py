@@ -65,11 +65,11 @@ class FusedQ1:
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
-    def trial_energy(self, u, ref, base=None):
+    def trial_energy(self, u, ref, base=None, out=None):
         """soft_energy(u, ref, base) and barrier_energy(u, base) from one sweep: [jd, curl, difference, barrier
         value, min J, unweighted barrier change, min base J], the same bits as the two calls."""
         self._check(u, ref) if base is None else self._check(u, ref, base)
-        out = torch.empty(7, dtype=torch.float64, device=self.device)
+        out = torch.empty(7, dtype=torch.float64, device=self.device) if out is None else out
         w, n = self._workspace(OP_TRIAL)
         _lib.call("seqvp_q1_trial_energy", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
                   *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
@@ -93,13 +93,13 @@ class FusedQ1:
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
-    def node_energy(self, warped, fixed, u, ref, lam, data_weight=None, base_warped=None, base_u=None):
+    def node_energy(self, warped, fixed, u, ref, lam, data_weight=None, base_warped=None, base_u=None, out=None):
         """[image, proximity, image difference vs base, proximity difference vs base]."""
         self._check(u, ref)
         for a in (warped, fixed):
             if a.dtype != torch.float64 or tuple(a.shape) != self.shape or not a.is_contiguous():
                 raise InputValidationError("node scalar fields must be contiguous float64 (nz, ny, nx)")
-        out = torch.empty(4, dtype=torch.float64, device=self.device)
+        out = torch.empty(4, dtype=torch.float64, device=self.device) if out is None else out
         w, n = self._workspace(OP_NODE)
         hx, hy, hz = self.grid.spacing
         _lib.call("seqvp_node_energy", _lib.ptr(warped), _lib.ptr(fixed), _lib.ptr(data_weight), _lib.ptr(u),

@@ -247,17 +247,16 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 # accelerated.py:1451; sequential.py:288-338)
                 st.outer_cache = st.outer_norms()
                 G = st.outer_cache[2]
+            elif metric_ready:
+                # the gradient, the direction and its scalars are enqueued before the gradient checks and read with
+                # the gradient norm (and the last step's ratio) in one synchronisation; when a check below ends the
+                # solve they are simply not used
+                scalars = st.direction_scalars(previous, norm=True, ratio=ratio_pending,
+                                               fp32=config.fp32_direction, gradient=True)
+                G = scalars[3]
             else:
                 st.gradient()
-                if metric_ready:
-                    # the direction and its scalars are enqueued before the gradient checks and read with the
-                    # gradient norm (and the last step's ratio) in one synchronisation; when a check below ends
-                    # the solve they are simply not used
-                    st.direction(fp32=config.fp32_direction)
-                    scalars = st.direction_scalars(previous, norm=True, ratio=ratio_pending)
-                    G = scalars[3]
-                else:
-                    G = st.gradient_norm()
+                G = st.gradient_norm()
         except FloatingPointError as exc:
             return finish("nonfinite_evaluation", str(exc))
         if ratio_pending:
@@ -286,10 +285,9 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 return finish("nonfinite_evaluation", str(exc))
             metric_ready = True
         if scalars is None:
-            st.direction(fp32=config.fp32_direction)
-            # the slope, the BB2 products and the domain bound come back together (the bound and the products
-            # are not used when the slope ends the solve)
-            scalars = st.direction_scalars(previous)
+            # the direction, the slope, the BB2 products and the domain bound come back together (the bound and
+            # the products are not used when the slope ends the solve)
+            scalars = st.direction_scalars(previous, fp32=config.fp32_direction)
         slope, products, bound = scalars[:3]
         if not np.isfinite(slope):
             return finish("nonfinite_evaluation", "PG directional derivative")

@@ -97,9 +97,10 @@ class DeviceSpline:
         return (_lib.ptr(self.c), cz, cy, cx, _lib.ptr(tz), tz.numel(), _lib.ptr(ty), ty.numel(), _lib.ptr(tx),
                 tx.numel())
 
-    def warp(self, u, *, with_gradient=True, value=None, gradient=None):
+    def warp(self, u, *, with_gradient=True, value=None, gradient=None, status=None):
         """W = M(X + u) and b = grad M(X + u) on the node grid. Returns (value, gradient, n_invalid);
-        n_invalid counts nodes outside the support (strict on free nodes), a device int64 tensor."""
+        n_invalid counts nodes outside the support (strict on free nodes), a device int64 tensor (`status` when
+        given, else the sampler's own)."""
         nz, ny, nx = self.grid.shape
         if u.dtype != torch.float64 or tuple(u.shape) != (nz, ny, nx, 3) or not u.is_contiguous():
             raise InputValidationError("u must be a contiguous float64 (nz, ny, nx, 3) device field")
@@ -108,10 +109,11 @@ class DeviceSpline:
             gradient = torch.empty_like(u)
         hx, hy, hz = self.grid.spacing
         ox, oy, oz = self.grid.origin
+        status = self.status if status is None else status
         _lib.call("seqvp_spline_warp", _lib.ptr(u), *self._spline_args(), ox, oy, oz, hx, hy, hz, nx, ny, nz,
-                  _lib.ptr(value), _lib.ptr(gradient if with_gradient else None), _lib.ptr(self.status),
+                  _lib.ptr(value), _lib.ptr(gradient if with_gradient else None), _lib.ptr(status),
                   _lib.stream_handle(self.device))
-        return value, (gradient if with_gradient else None), self.status
+        return value, (gradient if with_gradient else None), status
 
     def sample(self, points, *, chunk_size=65536, strict=None, with_gradient=True):
         """Value and xyz gradient at (..., 3) device points (TorchSpline.sample, backends/spline.py:81-127): every

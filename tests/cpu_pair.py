@@ -112,7 +112,11 @@ class CpuPairState:
             out.append(ojs.determinant_coefficients(A, V, self.m.delta))
         return np.concatenate(out)
 
-    def direction_scalars(self, prev, norm=False, ratio=False):
+    def direction_scalars(self, prev, norm=False, ratio=False, fp32=None, gradient=False):
+        if gradient:
+            self.gradient()
+        if fp32 is not None:
+            self.direction(fp32)
         (slope,) = self.dots((self.g, self.p))
         return (slope, (None if prev is None else self.bb2_products(prev)), self.step_bound(),
                 self.gradient_norm() if norm else None, self._ratio if ratio else None)

@@ -160,9 +160,9 @@ def _decode(v):
     return float.fromhex(v["hex"]) if isinstance(v, dict) else v
 
 
-def test_device_solver_follows_the_oracle_trajectory_for_50_outer_steps(gpu_device):
-    """The 32^3 CPU-sized pair (12 blobs, true JD >= 0.6), 50 certified outer steps from u = 0 with the reference's
-    defaults: the device driver on PairState against the oracle driver's trajectory stored by
+def test_device_solver_follows_the_stored_oracle_trajectory(gpu_device):
+    """The 32^3 CPU-sized pair (12 blobs, true JD >= 0.6), the stored number of certified outer steps from u = 0
+    with the reference's defaults: the device driver on PairState against the oracle driver's trajectory stored by
     tests/golden/make_driver_trajectory.py (every attempt's decision, tau and trial energy and every outer row to
     1e-12 relative; 97 entries of the final theta)."""
     import json

@@ -124,7 +124,7 @@ class _Scripted:
     def direction(self, fp32=False):
         pass
 
-    def direction_scalars(self, previous, norm=False, ratio=False):
+    def direction_scalars(self, previous, norm=False, ratio=False, fp32=None, gradient=False):
         return -1., None, 1e300, 1., None
 
     def witness_coefficients(self, cells, points):

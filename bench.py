@@ -79,6 +79,11 @@ def parse():
     ap.add_argument("--requests", type=int, default=24, help="serve workload: number of requests")
     ap.add_argument("--rate", type=float, default=2., help="serve workload: Poisson arrival rate (requests/s)")
     ap.add_argument("--slots", type=int, default=4, help="serve/batch workloads: concurrent requests")
+    ap.add_argument("--batched", action="store_true",
+                    help="serve/batch workloads: also run the requests with their trial-energy sweeps in multi-pair "
+                         "launches (batch.BatchGroup, launch shapes tuned online)")
+    ap.add_argument("--max-wait", type=float, default=.002,
+                    help="--batched: seconds the first pair of a launch waits for the others")
     ap.add_argument("--graphs", choices=("on", "off"), default="on",
                     help="trial and direction segments as captured HIP graphs (on) or issued call by call (off)")
     ap.add_argument("--start", choices=("zero", "half"), default="zero",
@@ -360,6 +365,27 @@ def serving_form(args, dev):
     torch.cuda.synchronize()
     phases["served"] = [t_a, time.monotonic_ns()]
     print(f"served {wall:.2f}s", file=sys.stderr, flush=True)
+    batched = None
+    if args.batched:
+        # the same requests again with every trial-energy sweep in the BatchGroup's multi-pair launches (tuned
+        # online: the first launches of each size class try the candidate shapes)
+        from seq_vp_3d_amd.batch import BatchGroup
+        group = BatchGroup(dev, max_wait=args.max_wait)
+        t_a = time.monotonic_ns()
+        bres, bwall, _ = serve(reqs, offsets, dev, slots=args.slots, batch=group)
+        torch.cuda.synchronize()
+        phases["batched"] = [t_a, time.monotonic_ns()]
+        print(f"batched {bwall:.2f}s, {group.launches} launches for {group.pairs_launched} pair sweeps",
+              file=sys.stderr, flush=True)
+        batched = dict(pairs_per_hour=round(len(reqs) / bwall * 3600, 1), wall_seconds=round(bwall, 3),
+                       speedup_vs_serial=round(serial_s / bwall, 3), speedup_vs_streams=round(wall / bwall, 3),
+                       launches=group.launches, pairs_per_launch=round(group.pairs_launched / max(group.launches, 1), 2),
+                       max_wait_s=args.max_wait,
+                       same_as_alone=all(a.theta_digest == b.theta_digest and a.energy == b.energy
+                                         for a, b in zip(alone, bres)),
+                       recertified=f"{sum(r.recertified == 'certified_feasible' for r in bres)}/{len(bres)}",
+                       latency_seconds={k: round(v, 4) for k, v in latency_summary(bres).items()},
+                       tuner={str(k): v for k, v in group.tuner.report().items()})
     same = all(a.theta_digest == b.theta_digest and a.energy == b.energy for a, b in zip(alone, mixed))
     for a, b in zip(alone, mixed):
         if a.theta_digest != b.theta_digest:
@@ -394,7 +420,7 @@ def serving_form(args, dev):
                 latency_seconds=dict(mixed={k: round(v, 4) for k, v in lat_mix.items()},
                                      alone={k: round(v, 4) for k, v in lat_alone.items()}),
                 max_concurrent=max(a[4] for a in server.admissions), recertified=f"{recert}/{n}",
-                same_alone_and_in_mix=same, phase_monotonic_ns=phases,
+                same_alone_and_in_mix=same, phase_monotonic_ns=phases, batched_launches=batched,
                 per_request=[dict(id=r.request_id, shape=list(r.shape), latency=round(r.latency, 4),
                                   solve=round(r.seconds["solve"], 4), setup=round(r.seconds["setup"], 4),
                                   recertify=round(r.seconds["recertify"], 4), status=r.status) for r in mixed])

@@ -53,6 +53,25 @@ int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base
                           int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                           size_t workspace_bytes, void* stream);
 
+/* One pair of a batched trial-energy launch: the arguments of seqvp_q1_trial_energy (base required). */
+typedef struct {
+  const double* u;
+  const double* ref;
+  const double* base;
+  const double* k;
+  int64_t nx, ny, nz;
+  double* out;               /* 7 device doubles */
+  void* workspace;           /* the pair's own workspace, op 10 */
+  size_t workspace_bytes;
+} SeqvpTrialBatchEntry;
+
+/* The trial energies of 1..32 pairs of any sizes in one persistent launch (the batching form; no counterpart in
+ * the reference, which evaluates one pair per call, triton_q1.py:572-609): the work list is every pair's 256-cell
+ * blocks, `tile` (1..64) consecutive blocks per work item, taken by `grid` workgroups (rounded up to a multiple of 8;
+ * 0 = one workgroup per item), then one reduction launch for all pairs. Each pair's out equals its
+ * seqvp_q1_trial_energy result bit for bit, whatever tile and grid. */
+int seqvp_q1_trial_energy_batch(const SeqvpTrialBatchEntry* pairs, int npairs, int tile, int64_t grid, void* stream);
+
 /* seqvp_q1_soft_energy without a workspace and without any allocation. No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). The block partials live in a fixed 4 MiB device array of the library (one per GPU, resolved on

@@ -8,6 +8,7 @@
 // stores; x-neighbour pairs in one wave are added by a lane shuffle and stored once), and one lane
 // per node adds them in a fixed order after the node base -- no atomics, so every result is
 // bit-reproducible.
+#include <algorithm>
 #include <mutex>
 
 #include "q1_common.h"
@@ -55,8 +56,8 @@ __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
 // One workgroup folds all block partials (stride 2*NS+NM) in a fixed order; out gets NS sums
 // (hi + lo) then NM minima, or, with INTERLEAVE (NS == NM), sum0, min0, sum1, min1, ...
 template <int NS, int NM, bool INTERLEAVE = false>
-__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
-                                                                double* __restrict__ out) {
+__device__ __forceinline__ void reduce_partials_body(const double* __restrict__ partials, int64_t nblocks,
+                                                     double* __restrict__ out) {
   constexpr int STRIDE = 2 * NS + NM;
   DD s[NS];
   double m[NM > 0 ? NM : 1];
@@ -100,6 +101,12 @@ __global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __
 #pragma unroll
     for (int q = 0; q < NM; ++q) out[INTERLEAVE ? 2 * q + 1 : NS + q] = fin[2 * NS + q];
   }
+}
+
+template <int NS, int NM, bool INTERLEAVE = false>
+__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
+                                                                double* __restrict__ out) {
+  reduce_partials_body<NS, NM, INTERLEAVE>(partials, nblocks, out);
 }
 
 __device__ __forceinline__ void load_ih(const double* __restrict__ k, double ih[3]) {
@@ -285,20 +292,19 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
 // The trial's soft and barrier energies in one sweep (the work of soft_energy_kernel and barrier_energy_kernel, in
 // the same per-cell order and the same blocks, so their partials are the same bits): the vertex loads and the edges
 // of the trial and its base are shared, and the closed-point pass runs on edges already in registers.
+// One 256-cell block `blk` of the sweep (the single-pair kernel and the batched persistent kernel share it); er_lds
+// is the workgroup's 36 * BLOCK doubles of LDS for the reference field's edges.
 template <bool WITH_BASE>
-__global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __restrict__ u,
-                                                             const double* __restrict__ ref,
-                                                             const double* __restrict__ base,
-                                                             const double* __restrict__ k, Dims d,
-                                                             double* __restrict__ soft_partials,
-                                                             double* __restrict__ barrier_partials) {
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void trial_energy_block(const int64_t blk, const double* __restrict__ u,
+                                                   const double* __restrict__ ref, const double* __restrict__ base,
+                                                   const double* __restrict__ k, const Dims& d,
+                                                   double* __restrict__ soft_partials,
+                                                   double* __restrict__ barrier_partials, double* er_lds) {
   const int64_t c = blk * BLOCK + threadIdx.x;
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   DD sb[2] = {{0.0, 0.0}, {0.0, 0.0}};
   double none[1] = {0.0};
   double m[2] = {__builtin_inf(), __builtin_inf()};
-  __shared__ double er_lds[36 * BLOCK];
   lds_double* er = (lds_double*)(er_lds + threadIdx.x);
   if (c < d.ncell) {
     double ih[3];
@@ -358,6 +364,74 @@ __global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __
   }
   block_reduce_store<3, 0>(s, none, soft_partials + blk * 6);
   block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
+}
+
+template <bool WITH_BASE>
+__global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __restrict__ u,
+                                                             const double* __restrict__ ref,
+                                                             const double* __restrict__ base,
+                                                             const double* __restrict__ k, Dims d,
+                                                             double* __restrict__ soft_partials,
+                                                             double* __restrict__ barrier_partials) {
+  __shared__ double er_lds[36 * BLOCK];
+  trial_energy_block<WITH_BASE>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d, soft_partials,
+                                barrier_partials, er_lds);
+}
+
+// ---- batched trial energies: many pairs in one persistent launch ------------------------------------------------
+// The work list is every pair's 256-cell blocks, `tile` consecutive blocks per work item; pair q's items start at
+// item_begin (a multiple of 8). Workgroup w of a grid that is a multiple of 8 takes items w, w + grid, ...: on MI355X
+// workgroups go to the 8 XCDs round robin, so all of w's items sit on XCD w mod 8 and the pair-local item index
+// has the same residue; xcd_block then hands each XCD a contiguous range of the pair's cells, as in the single-pair
+// launch. Each block writes the same partial slot with the same bits as trial_energy_kernel<true>, and the batched
+// reduction folds them in the same order, so every pair's 7 results are those of seqvp_q1_trial_energy.
+constexpr int TRIAL_BATCH_MAX = 32;
+struct TrialBatchPair {
+  const double* u;
+  const double* ref;
+  const double* base;
+  const double* k;
+  double* soft_partials;
+  double* barrier_partials;
+  double* out;
+  int32_t nx, ny, nz;
+  int32_t item_begin, nitems, nblocks;
+};
+struct TrialBatchTable {                 // passed by value (kernel arguments, < 4 KiB)
+  int32_t npairs, tile, nitems, pad;
+  TrialBatchPair p[TRIAL_BATCH_MAX];
+};
+static_assert(sizeof(TrialBatchTable) <= 4096, "kernel argument table");
+
+__device__ __forceinline__ int batch_pair_of(const TrialBatchTable& t, int32_t item) {
+  int q = 0;
+  for (int i = 1; i < t.npairs; ++i) q = item >= t.p[i].item_begin ? i : q;
+  return q;
+}
+
+__global__ void __launch_bounds__(BLOCK, 2) trial_energy_batch_kernel(const TrialBatchTable t) {
+  __shared__ double er_lds[36 * BLOCK];
+  for (int32_t item = blockIdx.x; item < t.nitems; item += gridDim.x) {
+    const TrialBatchPair& P = t.p[batch_pair_of(t, item)];
+    const int32_t li = item - P.item_begin;
+    if (li >= P.nitems) continue;                 // alignment padding between pairs
+    const Dims d = make_dims(P.nx, P.ny, P.nz);
+    const int64_t r = xcd_block(li, P.nitems);
+    for (int j = 0; j < t.tile; ++j) {
+      const int64_t blk = r * t.tile + j;
+      if (blk >= P.nblocks) break;
+      __syncthreads();                            // the previous block's reduction has read its LDS
+      trial_energy_block<true>(blk, P.u, P.ref, P.base, P.k, d, P.soft_partials, P.barrier_partials, er_lds);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) trial_energy_batch_reduce_kernel(const TrialBatchTable t) {
+  const TrialBatchPair& P = t.p[blockIdx.x];
+  if (blockIdx.y == 0)
+    reduce_partials_body<3, 0>(P.soft_partials, P.nblocks, P.out);
+  else
+    reduce_partials_body<2, 2, true>(P.barrier_partials, P.nblocks, P.out + 3);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1571,6 +1645,47 @@ extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const d
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
   hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, soft, nb, out);
   hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, bar, nb, out + 3);
+  return launch_status();
+}
+
+extern "C" int seqvp_q1_trial_energy_batch(const SeqvpTrialBatchEntry* pairs, int npairs, int tile, int64_t grid,
+                                           void* stream) {
+  if (!pairs || npairs < 1 || npairs > TRIAL_BATCH_MAX || tile < 1 || tile > 64 || grid < 0) return SEQVP_EINVAL;
+  TrialBatchTable t = {};
+  t.npairs = npairs;
+  t.tile = tile;
+  int64_t items = 0;
+  for (int i = 0; i < npairs; ++i) {
+    const SeqvpTrialBatchEntry& e = pairs[i];
+    if (!e.u || !e.ref || !e.base || !e.k || !e.out || !grid_ok(e.nx, e.ny, e.nz)) return SEQVP_EINVAL;
+    if (e.nx > INT32_MAX || e.ny > INT32_MAX || e.nz > INT32_MAX) return SEQVP_EINVAL;
+    if (!e.workspace || e.workspace_bytes < q1_workspace(10, e.nx, e.ny, e.nz)) return SEQVP_EWORKSPACE;
+    const Dims d = make_dims(e.nx, e.ny, e.nz);
+    const int64_t nb = cdiv(d.ncell, BLOCK);
+    const int64_t ni = cdiv(nb, tile);
+    TrialBatchPair& P = t.p[i];
+    P.u = e.u;
+    P.ref = e.ref;
+    P.base = e.base;
+    P.k = e.k;
+    P.soft_partials = (double*)e.workspace;
+    P.barrier_partials = P.soft_partials + nb * 6;
+    P.out = e.out;
+    P.nx = (int32_t)e.nx;
+    P.ny = (int32_t)e.ny;
+    P.nz = (int32_t)e.nz;
+    P.item_begin = (int32_t)items;
+    P.nitems = (int32_t)ni;
+    P.nblocks = (int32_t)nb;
+    items += cdiv(ni, 8) * 8;
+    if (items > INT32_MAX || nb > INT32_MAX) return SEQVP_EINVAL;
+  }
+  t.nitems = (int32_t)items;
+  // a persistent grid is a multiple of 8 (items w, w + grid, ... then share w's XCD); 0 = one workgroup per item
+  const int64_t g = grid == 0 ? items : std::min<int64_t>(cdiv(grid, 8) * 8, items);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(trial_energy_batch_kernel, dim3(g), dim3(BLOCK), 0, s, t);
+  hipLaunchKernelGGL(trial_energy_batch_reduce_kernel, dim3(npairs, 2), dim3(BLOCK), 0, s, t);
   return launch_status();
 }
 

@@ -81,7 +81,7 @@ class PairState:
     are issued one by one. Both end with one copy of the segment's scalars to page-locked host memory and one wait."""
 
     def __init__(self, problem, fixed, moving, device="cuda", data_weight=None, sampler=None, certificate=None,
-                 graphs=True):
+                 graphs=True, batch=None):
         self.problem = problem
         self.grid = problem.grid
         self.device = torch.device(device)
@@ -154,6 +154,8 @@ class PairState:
         sizes.append(_lib.raw("seqvp_node_min_jd_extended_workspace_bytes", nx, ny, nz))
         self.q1.ws.get(max(sizes))
         self.graphs = SegmentGraphs(self.device) if graphs else None
+        # batching form: the trial-energy sweep goes through this BatchGroup's multi-pair launches
+        self.batch = batch
 
     def _segment(self, key, body):
         """Runs a device-only segment: its captured graph for `key`, or its calls one by one without graphs."""
@@ -343,16 +345,23 @@ class PairState:
         # the warp (value and gradient at once: the 64-tap gather dominates the warp, and an accepted trial then
         # needs no second warp at promotion), the energies, and one copy of the 17 scalars to the host
         self._tau_np[0] = tau
-        key = key_of("trial", self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t, self.warped_t,
-                     self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided, self.cert.capacity)
-        self._segment(key, self._trial_segment)
+        batched = self.batch is not None
+        key = key_of("trial", batched, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
+                     self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
+                     self.cert.capacity)
+        self._segment(key, lambda: self._trial_segment(not batched))
+        if batched:
+            # the soft and barrier sweep in the group's multi-pair launch, then the scalars' copy
+            self.batch.trial_energy(self, self._trial_dev[4:11])
+            _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(self._trial_dev),
+                      17 * 8, self._stream())
         self.cert._u = self.u_t
         self._wait()
         vals = self._trial_np.tolist()
         nbad = int(self._trial_np[11:12].view(np.int64)[0])
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
 
-    def _trial_segment(self):
+    def _trial_segment(self, energies=True):
         nz, ny, nx = self.shape
         q, w, s = self.q1, self.problem.weights, self._stream()
         d = self._trial_dev
@@ -363,8 +372,9 @@ class PairState:
         self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t, status=d[11:12].view(torch.int64))
         q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
                       base_warped=self.warped, base_u=self.u, out=d[0:4])
-        q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11])      # soft and barrier terms in one sweep
-        _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
+        if energies:
+            q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11])      # soft and barrier terms in one sweep
+            _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
 
     def trial_energies(self, screen=False):
         """Energies and stable differences of the current candidate (u_t, warped_t) against the accepted state,

@@ -91,9 +91,10 @@ def _download(t):
     return host
 
 
-def run_request(req, device, stream=None, recert_workers=8):
+def run_request(req, device, stream=None, recert_workers=8, batch=None):
     """Runs one request on `stream` (the current stream when None): upload, device spline, certified solver from
-    u = 0, CPU re-certification of the output. Times are time.perf_counter values."""
+    u = 0, CPU re-certification of the output. With `batch` (a batch.BatchGroup) the trial-energy sweeps go through
+    the group's multi-pair launches. Times are time.perf_counter values."""
     started = time.perf_counter()
     ctx = torch.cuda.stream(stream) if stream is not None else torch.cuda.stream(torch.cuda.current_stream(device))
     secs = {}
@@ -101,14 +102,20 @@ def run_request(req, device, stream=None, recert_workers=8):
         nz, ny, nx = req.shape
         grid = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=1., hy=1., hz=1.)
         fixed, moving = (_upload(a, device) for a in (req.fixed, req.moving))
-        st = PairState(Q1Problem(grid, native_weights(grid.shape), 0., .05), fixed, moving, device)
+        st = PairState(Q1Problem(grid, native_weights(grid.shape), 0., .05), fixed, moving, device, batch=batch)
         st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=device))
         t1 = time.perf_counter()
         secs["setup"] = t1 - started
         trajectory = []
-        res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=req.jd_step_global),
-                               max_outer=req.max_outer,
-                               inner_observer=lambda r: trajectory.append((r["reason"], r["tau"], r["energy_trial"])))
+        if batch is not None:
+            batch.join()
+        try:
+            res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=req.jd_step_global),
+                                   max_outer=req.max_outer,
+                                   inner_observer=lambda r: trajectory.append((r["reason"], r["tau"], r["energy_trial"])))
+        finally:
+            if batch is not None:
+                batch.leave()
         t2 = time.perf_counter()
         secs["solve"] = t2 - t1
         # downloads into page-locked buffers (torch's host cache reuses them across requests), one wait
@@ -132,8 +139,9 @@ class Server:
     """Continuous-batching request loop on one device: `slots` worker threads, each with its own stream, admitting
     requests first-fit in arrival order while their footprints fit `hbm_budget` bytes."""
 
-    def __init__(self, device="cuda", slots=4, hbm_budget=None, recert_workers=4):
+    def __init__(self, device="cuda", slots=4, hbm_budget=None, recert_workers=4, batch=None):
         self.device = torch.device(device)
+        self.batch = batch              # batch.BatchGroup shared by the slots (None: each pair launches alone)
         self.slots = int(slots)
         if hbm_budget is None:
             free, _total = torch.cuda.mem_get_info(self.device)
@@ -190,7 +198,7 @@ class Server:
                 self._running += 1
                 self.admissions.append((time.perf_counter(), req.request_id, need, self._used, self._running))
             try:
-                fut.set_result(run_request(req, self.device, stream, self.recert_workers))
+                fut.set_result(run_request(req, self.device, stream, self.recert_workers, self.batch))
             except BaseException as exc:          # a failed request never takes the server down
                 fut.set_exception(exc)
             finally:
@@ -207,10 +215,11 @@ class Server:
             t.join()
 
 
-def serve(requests, offsets=None, device="cuda", slots=4, hbm_budget=None, recert_workers=4):
+def serve(requests, offsets=None, device="cuda", slots=4, hbm_budget=None, recert_workers=4, batch=None):
     """Submit the requests at their arrival offsets (seconds from the start; None = all at once, the batching
-    form) and collect their results. Returns (results in request order, wall seconds, server)."""
-    server = Server(device, slots=slots, hbm_budget=hbm_budget, recert_workers=recert_workers)
+    form) and collect their results. `batch`: a batch.BatchGroup for multi-pair trial launches. Returns (results
+    in request order, wall seconds, server)."""
+    server = Server(device, slots=slots, hbm_budget=hbm_budget, recert_workers=recert_workers, batch=batch)
     t0 = time.perf_counter()
     futures = []
     for i, req in enumerate(requests):

@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--inner", type=int, default=5)
     ap.add_argument("--shape", type=str, default="160,192,224")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="driver workload: skip the batching-form measurement added to the line after the timed "
+                         "region")
     ap.add_argument("--workload", choices=("driver", "fixed", "precision", "serve", "batch", "file"),
                     default="driver")
     ap.add_argument("--requests", type=int, default=24, help="serve workload: number of requests")
@@ -325,6 +328,11 @@ def serving_form(args, dev):
     execution of the same requests; each request runs `--outer` certified outer steps from u = 0 and its output is
     re-certified on the CPU. One JSON line: certified pairs/hour of both, p50/p99 latency, and whether every
     result is bitwise the same alone and in the mix."""
+    print(json.dumps(forms_line(args, dev)), flush=True)
+
+
+def forms_line(args, dev):
+    """The measurement of serving_form as a dict."""
     import numpy as np
     import torch
 
@@ -424,7 +432,7 @@ def serving_form(args, dev):
                 per_request=[dict(id=r.request_id, shape=list(r.shape), latency=round(r.latency, 4),
                                   solve=round(r.seconds["solve"], 4), setup=round(r.seconds["setup"], 4),
                                   recertify=round(r.seconds["recertify"], 4), status=r.status) for r in mixed])
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def main():
@@ -563,6 +571,23 @@ def main():
     )
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(shape, args.inner)
+    if args.workload == "driver" and not args.no_forms and shape == CONFIG_NATIVE.shape:
+        # the batching form (24 pairs of uneven sizes, each to a certified result after 10 outer steps) beside the
+        # plain path, after the timed region: certified pairs/hour served, batched and serial, p50/p99 latency
+        del st, pair
+        torch.cuda.empty_cache()
+        import types
+        fargs = types.SimpleNamespace(workload="batch", outer=10, slots=args.slots, batched=True,
+                                      max_wait=args.max_wait, requests=24, rate=args.rate)
+        f = forms_line(fargs, dev)
+        line["forms"] = dict(
+            batch=dict(metric=f["metric"], served_pairs_per_hour=f["value"],
+                       serial_pairs_per_hour=f["serial_pairs_per_hour"], speedup_vs_serial=f["speedup_vs_serial"],
+                       batched_launch=f["batched_launches"], latency_seconds=f["latency_seconds"],
+                       sizes=f["config"]["sizes"], outer_steps_per_request=10, recertified=f["recertified"],
+                       same_alone_and_in_mix=f["same_alone_and_in_mix"],
+                       final_energy_max_rel_diff_vs_alone=f["final_energy_max_rel_diff_vs_alone"]),
+            serving="bench.py --workload serve --requests 200 --rate 2 (about two minutes of arrivals)")
     print(json.dumps(line), flush=True)
 
 

@@ -9,7 +9,10 @@ the BB history change places by swapping buffers, so a pair cycles through a few
 graph (least recently used first out).
 
 The null stream cannot be captured: capture runs on a private stream of the segment set (nothing executes while
-capturing), and launches go to the caller's current stream.
+capturing), and launches go to the caller's current stream. Capture only where no other host thread uses the device
+at the same time: while a capture is open, the HIP runtime fails blocking copies and synchronisations of other
+threads with hipErrorStreamCaptureUnsupported, even in thread-local capture mode (the multi-slot server therefore
+issues its segments call by call).
 """
 import ctypes
 import threading

@@ -91,10 +91,11 @@ def _download(t):
     return host
 
 
-def run_request(req, device, stream=None, recert_workers=8, batch=None):
+def run_request(req, device, stream=None, recert_workers=8, batch=None, graphs=True):
     """Runs one request on `stream` (the current stream when None): upload, device spline, certified solver from
     u = 0, CPU re-certification of the output. With `batch` (a batch.BatchGroup) the trial-energy sweeps go through
-    the group's multi-pair launches. Times are time.perf_counter values."""
+    the group's multi-pair launches. `graphs`: the pair's trial and direction segments as captured HIP graphs (only
+    where no other host thread uses the device, see Server). Times are time.perf_counter values."""
     started = time.perf_counter()
     ctx = torch.cuda.stream(stream) if stream is not None else torch.cuda.stream(torch.cuda.current_stream(device))
     secs = {}
@@ -102,7 +103,8 @@ def run_request(req, device, stream=None, recert_workers=8, batch=None):
         nz, ny, nx = req.shape
         grid = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=1., hy=1., hz=1.)
         fixed, moving = (_upload(a, device) for a in (req.fixed, req.moving))
-        st = PairState(Q1Problem(grid, native_weights(grid.shape), 0., .05), fixed, moving, device, batch=batch)
+        st = PairState(Q1Problem(grid, native_weights(grid.shape), 0., .05), fixed, moving, device, batch=batch,
+                       graphs=graphs)
         st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=device))
         t1 = time.perf_counter()
         secs["setup"] = t1 - started
@@ -137,7 +139,12 @@ def run_request(req, device, stream=None, recert_workers=8, batch=None):
 
 class Server:
     """Continuous-batching request loop on one device: `slots` worker threads, each with its own stream, admitting
-    requests first-fit in arrival order while their footprints fit `hbm_budget` bytes."""
+    requests first-fit in arrival order while their footprints fit `hbm_budget` bytes.
+
+    The slots issue their segments call by call (PairState(graphs=False)): while one host thread captures a HIP
+    graph, the HIP runtime fails blocking copies and synchronisations issued by other threads with
+    hipErrorStreamCaptureUnsupported ("operation not permitted when stream is capturing"), even in thread-local
+    capture mode, and the slots synchronise all the time. Results are the same bits either way."""
 
     def __init__(self, device="cuda", slots=4, hbm_budget=None, recert_workers=4, batch=None):
         self.device = torch.device(device)
@@ -198,7 +205,7 @@ class Server:
                 self._running += 1
                 self.admissions.append((time.perf_counter(), req.request_id, need, self._used, self._running))
             try:
-                fut.set_result(run_request(req, self.device, stream, self.recert_workers, self.batch))
+                fut.set_result(run_request(req, self.device, stream, self.recert_workers, self.batch, graphs=False))
             except BaseException as exc:          # a failed request never takes the server down
                 fut.set_exception(exc)
             finally:

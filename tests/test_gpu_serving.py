@@ -21,13 +21,24 @@ def _requests(gpu_device, shapes, max_outer=3):
 SHAPES = [(24, 26, 28), (20, 22, 24), (28, 24, 20), (22, 22, 22)]
 
 
-def test_results_are_the_same_alone_and_in_the_mix(gpu_device):
+def test_results_are_the_same_alone_and_in_the_mix(gpu_device, monkeypatch):
+    from seq_vp_3d_amd import serving
     from seq_vp_3d_amd.serving import serial, serve
     reqs = _requests(gpu_device, SHAPES)
     print("case serving serial", flush=True)
     alone, _ = serial(reqs, gpu_device)
     print("case serving mix", flush=True)
+    # the slots never capture graphs: a capture open on one thread fails the other threads' blocking copies
+    seen = []
+    original = serving.run_request
+
+    def recording(*args, **kw):
+        seen.append(kw.get("graphs", True))
+        return original(*args, **kw)
+
+    monkeypatch.setattr(serving, "run_request", recording)
     mixed, _, server = serve(reqs, None, gpu_device, slots=3)
+    assert seen == [False] * len(reqs)
     assert max(a[4] for a in server.admissions) >= 2          # requests did overlap on the device
     for a, b in zip(alone, mixed):
         assert a.theta_digest == b.theta_digest and a.energy == b.energy and a.status == b.status

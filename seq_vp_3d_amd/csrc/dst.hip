@@ -11,10 +11,10 @@
 // third pass's stores and the post-scale/negation into the last pass's stores: 6 passes over the
 // 3F-vector, ping-ponging through one workspace buffer.
 //
-// Tiles: one workgroup = 4 wave64 computes a 64x64 output tile; each wave owns a 32x32 quadrant as
-// 2x2 v_mfma_f64_16x16x4f64 accumulators. Operand fragments of the data matrix are staged through
-// LDS in 64x16 (or 16x64) k-slabs with coalesced global loads; S fragments are read from L2
-// (S is at most 222^2 doubles).
+// Tiles: lengths up to 256 use the folded strip kernels below (a workgroup owns 64 lines of the independent index and
+// one wave per 16 outputs, v_mfma_f64_16x16x4f64); longer axes use a 64x64 output tile per workgroup of 4 wave64,
+// each wave a 32x32 quadrant as 2x2 accumulators. Operand fragments of the data matrix are staged through LDS in
+// 64x16 (or 16x64) k-slabs with coalesced global loads; S fragments are read from L2.
 #include <math.h>
 
 #include "q1_common.h"
@@ -211,61 +211,67 @@ __global__ void __launch_bounds__(256) dst_cols_kernel(const T* __restrict__ X, 
       }
 }
 
-// ---- folded full-width passes (n <= 256) -------------------------------------------------------------------
+// ---- folded passes (n <= 256) ------------------------------------------------------------------------------------
 // The DST-I matrix satisfies S[n-1-j][k] = (-1)^k S[j][k]. Pairing j with n-1-j, an output k is
 //   even k: sum_{j < h'} U[j] S[j][k],  U[j] = x[j] + x[n-1-j] (j < h), U[h] = x[h] when n is odd,
 //   odd k:  sum_{j < h}  V[j] S[j][k],  V[j] = x[j] - x[n-1-j],
 // with h = floor(n/2), h' = ceil(n/2): two half-length contractions, half the MFMA work of the plain one.
-//
-// One workgroup of 2 NH waves (NH = ceil(h'/64)) owns a 64-wide tile of the independent index and every output
-// of it: waves 0..NH-1 compute the even outputs 2c, waves NH..2NH-1 the odd outputs 2c+1, c in [64w', 64w'+64),
-// as 4x4 v_mfma tiles, so the input is read from HBM once. U and V are formed while a k-slab (FK = 16 folded
-// indices) is written to LDS, double-buffered: the next slab's global loads are in flight in registers while
-// the waves run the current slab's 64 MFMAs each, one barrier per slab. Every load is unconditional from a
-// clamped in-bounds address (zero padding by select), the sine fragments of a slab are issued before its first
-// MFMA, and the folded sine matrix Sf (KP x LDF, zero-padded: Sf[j][c] = S[j][2c] for c < 64 NH, S[j][2c'+1]
-// with c' = c - 64 NH after) needs no bounds test, so no wait sits between a load and the MFMAs it overlaps.
+// U and V are formed while a k-slab (FK = 16 folded indices) is written to LDS, double-buffered: the next slab's
+// global loads are in flight in registers while the waves run the current slab's MFMAs, one barrier per slab. Every
+// load is unconditional from a clamped in-bounds address (zero padding by select), the sine fragments of a slab are
+// issued before its first MFMA, and the sine matrix is stored zero-padded, so no wait sits between a load and the
+// MFMAs it overlaps.
 constexpr int FK = 16;
 
 __device__ __forceinline__ int64_t clamp_index(int64_t i, int64_t n) { return i < n ? i : n - 1; }
 
-// folded sine matrix of length n: KP rows (folded index j), LDF = 128 NH columns (even then odd outputs)
+// Strip tiling: the outputs are cut into 16-wide strips: ne = ceil(ceil(n/2)/16) strips of even
+// outputs, then no = ceil(floor(n/2)/16) strips of odd outputs. One workgroup of NW = max(ne + no, 2) waves owns a
+// 64-wide tile of the independent index, and wave w computes strip w against the whole tile (four v_mfma
+// accumulators), so the MFMA work is that of ceil(n/2) outputs rounded up to 16 per side (lengths 222/190/158:
+// 224/192/160 output rows; a 64-output tiling per wave would compute 256). Per apply at 160x192x224: 0.92 ms against
+// 0.95 ms for the 64-output tiling (the passes are not MFMA-bound: 2-2.5 TB/s of HBM at 4-5 waves per SIMD). The
+// strip sine matrix Sst (KP x 16 NW) holds, in column 16 s + i, output i of strip s (Sst[j][16 s + i] = S[j][k] with
+// k = 2(16 s + i) for an even strip, k = 2(16 (s - ne) + i) + 1 for an odd one; zero outside).
+constexpr int SW = 16;
+
 template <typename T>
-__global__ void fold_sine_kernel(T* __restrict__ Sf, int64_t n, int64_t KP, int64_t LDF) {
+__global__ void strip_sine_kernel(T* __restrict__ Sst, int64_t n, int64_t KP, int64_t LDS, int64_t ne) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= KP * LDF) return;
-  const int64_t j = i / LDF, c = i % LDF, half = LDF / 2;
+  if (i >= KP * LDS) return;
+  const int64_t j = i / LDS, col = i % LDS, s = col / SW;
   const int64_t hp = (n + 1) / 2, h = n / 2;
-  const bool odd = c >= half;
-  const int64_t cc = odd ? c - half : c, k = odd ? 2 * cc + 1 : 2 * cc;
-  if (odd ? (j >= h || cc >= h) : (j >= hp || cc >= hp)) {
-    Sf[i] = (T)0;
+  const bool odd = s >= ne;
+  const int64_t c = (odd ? s - ne : s) * SW + col % SW, k = odd ? 2 * c + 1 : 2 * c;
+  if (odd ? (j >= h || c >= h) : (j >= hp || c >= hp)) {
+    Sst[i] = (T)0;
     return;
   }
   const int64_t m = ((j + 1) * (k + 1)) % (2 * (n + 1));
-  Sf[i] = (T)(sqrt(2.0 / (double)(n + 1)) * sinpi((double)m / (double)(n + 1)));
+  Sst[i] = (T)(sqrt(2.0 / (double)(n + 1)) * sinpi((double)m / (double)(n + 1)));
 }
 
-// x axis: Out[o][k] = sum_j X[o][j] S[j][k] for 64 rows o per workgroup.
-template <typename T, typename TIN, typename TOUT, int PRE, int POST, int NH>
-__global__ void __launch_bounds__(128 * NH, 2) dst_rows_fold_kernel(const TIN* __restrict__ X, const double* __restrict__ pre,
-                                                                    const T* __restrict__ Sf, int64_t LDF, int64_t O,
-                                                                    int64_t n, const double* __restrict__ post,
-                                                                    double sign, TOUT* __restrict__ Y) {
+// x axis: Out[o][k] = sum_j X[o][j] S[j][k] for 64 rows o per workgroup. PER = slab elements per thread (a power of
+// two with PER * blockDim.x >= 64 * FK).
+template <typename T, typename TIN, typename TOUT, int PRE, int POST, int PER>
+__global__ void __launch_bounds__(1024) dst_rows_strip_kernel(const TIN* __restrict__ X, const double* __restrict__ pre,
+                                                              const T* __restrict__ Sst, int64_t LDS, int64_t ne,
+                                                              int64_t nstrip, int64_t O, int64_t n,
+                                                              const double* __restrict__ post, double sign,
+                                                              TOUT* __restrict__ Y) {
   static_assert(POST != POST_DIV, "the spectral division is fused into a strided pass");
   typedef typename Acc<T>::type AT;
-  constexpr int NT = 128 * NH, PER = (TM * FK + NT - 1) / NT;
+  const int NT = blockDim.x;
   __shared__ T As[2][2][FK][TM + 1];   // As[buffer][U/V][k][m]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int side = wave >= NH ? 1 : 0;             // 0 even outputs (U), 1 odd outputs (V)
+  const bool active = wave < nstrip;                   // wave-uniform: a padding wave only stages
+  const int side = wave >= ne ? 1 : 0;                 // 0 even outputs (U), 1 odd outputs (V)
+  const int64_t c0 = SW * (wave - side * ne);
   const int64_t m0 = xcd_block(blockIdx.x, gridDim.x) * TM;
-  const int c0 = 64 * (wave - side * NH);
   const int64_t h = n / 2, hp = (n + 1) / 2;
-  AT acc[4][4];
+  AT acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  for (int a = 0; a < 4; ++a) acc[a] = AT{0, 0, 0, 0};
   const int li = lane & 15, lk = lane >> 4;
   TIN xa[PER], xb[PER];
   double pa[PER], pb[PER];
@@ -286,6 +292,7 @@ __global__ void __launch_bounds__(128 * NH, 2) dst_rows_fold_kernel(const TIN* _
   };
   fetch(0);
   int buf = 0;
+  const int64_t scol = SW * (int64_t)(active ? wave : 0) + li;
   for (int64_t k0 = 0; k0 < hp; k0 += FK) {
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
@@ -301,83 +308,72 @@ __global__ void __launch_bounds__(128 * NH, 2) dst_rows_fold_kernel(const TIN* _
       }
     }
     __syncthreads();
-    // sine fragments first, then the next slab: the memory counter retires loads in order, so the MFMAs wait
-    // only for the (L2) fragments while the slab's HBM loads stay in flight
-    T bf[FK / 4][4];
+    T bf[FK / 4];
 #pragma unroll
-    for (int q = 0; q < FK / 4; ++q)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bf[q][b] = Sf[(k0 + 4 * q + lk) * LDF + 64 * wave + 16 * b + li];
+    for (int q = 0; q < FK / 4; ++q) bf[q] = Sst[(k0 + 4 * q + lk) * LDS + scol];
     fetch(k0 + FK);   // unconditional (clamped past the end) so the wait counts stay static
-    __builtin_amdgcn_sched_barrier(0);   // keep the slab's loads ahead of the MFMAs they overlap
+    __builtin_amdgcn_sched_barrier(0);
+    if (active) {
 #pragma unroll
-    for (int q = 0; q < FK / 4; ++q) {
+      for (int q = 0; q < FK / 4; ++q) {
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const T af = As[buf][side][4 * q + lk][16 * a + li];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = mfma(af, bf[q][b], acc[a][b]);
+        for (int a = 0; a < 4; ++a) acc[a] = mfma(As[buf][side][4 * q + lk][16 * a + li], bf[q], acc[a]);
       }
     }
     buf ^= 1;
   }
-  // epilogue, one 16-row band at a time: its 16 scale values are loaded (clamped, unconditionally) before any
-  // store, so the stores do not each wait for a load
+  if (!active) return;
   const int64_t nout = side ? h : hp;
+  const int64_t c = c0 + li;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    double pv[4][4];
+    double pv[4];
     if (POST == POST_SCALE) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r), c = c0 + 16 * b + li;
-          pv[b][r] = post ? post[clamp_index(gm, O) * n + clamp_index(2 * c + side, n)] : 1.0;
-        }
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r);
+        pv[r] = post ? post[clamp_index(gm, O) * n + clamp_index(2 * c + side, n)] : 1.0;
+      }
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r), c = c0 + 16 * b + li;
-        if (gm < O && c < nout) {
-          const int64_t idx = gm * n + 2 * c + side;
-          if (POST == POST_SCALE) {
-            const double v = (double)acc[a][b][r];
-            Y[idx] = (TOUT)(sign * (post ? pv[b][r] * v : v));
-          } else {
-            Y[idx] = (TOUT)acc[a][b][r];
-          }
+    for (int r = 0; r < 4; ++r) {
+      const int64_t gm = m0 + 16 * a + acc_row<T>(lk, r);
+      if (gm < O && c < nout) {
+        const int64_t idx = gm * n + 2 * c + side;
+        if (POST == POST_SCALE) {
+          const double v = (double)acc[a][r];
+          Y[idx] = (TOUT)(sign * (post ? pv[r] * v : v));
+        } else {
+          Y[idx] = (TOUT)acc[a][r];
         }
       }
+    }
   }
 }
 
-// strided axes: Out_o[k][i] = sum_j S[k][j] X_o[j][i] for 64 inner columns i per workgroup (S symmetric, so the
-// A fragment S[k][j] is Sf's entry for folded row j and output k).
-template <typename T, int POST, int NH>
-__global__ void __launch_bounds__(128 * NH, 2) dst_cols_fold_kernel(const T* __restrict__ X, const T* __restrict__ Sf,
-                                                                    int64_t LDF, int64_t n, int64_t I,
-                                                                    const double* __restrict__ den, T* __restrict__ Y) {
+// strided axes: Out_o[k][i] = sum_j S[k][j] X_o[j][i] for 64 inner columns i per workgroup.
+template <typename T, int POST, int PER>
+__global__ void __launch_bounds__(1024) dst_cols_strip_kernel(const T* __restrict__ X, const T* __restrict__ Sst,
+                                                              int64_t LDS, int64_t ne, int64_t nstrip, int64_t n,
+                                                              int64_t I, const double* __restrict__ den,
+                                                              T* __restrict__ Y) {
   typedef typename Acc<T>::type AT;
-  constexpr int NT = 128 * NH, PER = (FK * TN + NT - 1) / NT;
+  const int NT = blockDim.x;
   __shared__ T Bs[2][2][FK][TN + 1];   // Bs[buffer][U/V][k][i]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int side = wave >= NH ? 1 : 0;
+  const bool active = wave < nstrip;
+  const int side = wave >= ne ? 1 : 0;
+  const int64_t c0 = SW * (wave - side * ne);
   const int64_t nib = (I + TN - 1) / TN;
   const int64_t tile = xcd_block(blockIdx.x, gridDim.x);
   const int64_t o = tile / nib, n0 = (tile % nib) * TN;
   const T* Xo = X + o * n * I;
   T* Yo = Y + o * n * I;
-  const int r0 = 64 * (wave - side * NH);
   const int64_t h = n / 2, hp = (n + 1) / 2;
-  AT acc[4][4];
+  AT acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = AT{0, 0, 0, 0};
+  for (int b = 0; b < 4; ++b) acc[b] = AT{0, 0, 0, 0};
   const int li = lane & 15, lk = lane >> 4;
   T xa[PER], xb[PER];
   auto fetch = [&](int64_t k0) {
@@ -392,63 +388,54 @@ __global__ void __launch_bounds__(128 * NH, 2) dst_cols_fold_kernel(const T* __r
   };
   fetch(0);
   int buf = 0;
+  const int64_t scol = SW * (int64_t)(active ? wave : 0) + li;
   for (int64_t k0 = 0; k0 < hp; k0 += FK) {
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int e = t * NT + threadIdx.x;
       if (e < FK * TN) {
-        const int kk = e / TN, c = e % TN;
+        const int kk = e / TN, cc = e % TN;
         const int64_t j = k0 + kk;
-        const bool live = n0 + c < I;
-        Bs[buf][0][kk][c] = (live && j < h) ? xa[t] + xb[t] : ((live && j < hp) ? xa[t] : (T)0);
-        Bs[buf][1][kk][c] = (live && j < h) ? xa[t] - xb[t] : (T)0;
+        const bool live = n0 + cc < I;
+        Bs[buf][0][kk][cc] = (live && j < h) ? xa[t] + xb[t] : ((live && j < hp) ? xa[t] : (T)0);
+        Bs[buf][1][kk][cc] = (live && j < h) ? xa[t] - xb[t] : (T)0;
       }
     }
     __syncthreads();
-    T af[FK / 4][4];
+    T af[FK / 4];
 #pragma unroll
-    for (int q = 0; q < FK / 4; ++q)
-#pragma unroll
-      for (int a = 0; a < 4; ++a) af[q][a] = Sf[(k0 + 4 * q + lk) * LDF + 64 * wave + 16 * a + li];
+    for (int q = 0; q < FK / 4; ++q) af[q] = Sst[(k0 + 4 * q + lk) * LDS + scol];
     fetch(k0 + FK);   // unconditional (clamped past the end) so the wait counts stay static
-    __builtin_amdgcn_sched_barrier(0);   // keep the slab's loads ahead of the MFMAs they overlap
+    __builtin_amdgcn_sched_barrier(0);
+    if (active) {
 #pragma unroll
-    for (int q = 0; q < FK / 4; ++q) {
+      for (int q = 0; q < FK / 4; ++q) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const T bf = Bs[buf][side][4 * q + lk][16 * b + li];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) acc[a][b] = mfma(af[q][a], bf, acc[a][b]);
+        for (int b = 0; b < 4; ++b) acc[b] = mfma(af[q], Bs[buf][side][4 * q + lk][16 * b + li], acc[b]);
       }
     }
     buf ^= 1;
   }
-  // epilogue per 16-row band; with POST_DIV the band's 16 denominators are loaded (clamped) before its stores
+  if (!active) return;
   const int64_t nout = side ? h : hp;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    T dv[4][4];
+  for (int r = 0; r < 4; ++r) {
+    const int64_t c = c0 + acc_row<T>(lk, r);
+    T dv[4];
     if (POST == POST_DIV) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t c = r0 + 16 * a + acc_row<T>(lk, r), gi = n0 + 16 * b + li;
-          dv[b][r] = (T)den[clamp_index(2 * c + side, n) * I + clamp_index(gi, I)];
-        }
+      for (int b = 0; b < 4; ++b) dv[b] = (T)den[clamp_index(2 * c + side, n) * I + clamp_index(n0 + 16 * b + li, I)];
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t c = r0 + 16 * a + acc_row<T>(lk, r), gi = n0 + 16 * b + li;
-        if (c < nout && gi < I) {
-          T v = acc[a][b][r];
-          if (POST == POST_DIV) v = v / dv[b][r];
-          Yo[(2 * c + side) * I + gi] = v;
-        }
+    for (int b = 0; b < 4; ++b) {
+      const int64_t gi = n0 + 16 * b + li;
+      if (c < nout && gi < I) {
+        T v = acc[b][r];
+        if (POST == POST_DIV) v = v / dv[b];
+        Yo[(2 * c + side) * I + gi] = v;
       }
+    }
   }
 }
 
@@ -496,13 +483,20 @@ __global__ void __launch_bounds__(256) poisson_denominator_kernel(const double* 
 using namespace seqvp;
 
 static inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
-// Folded sine matrix of length n <= 256: NH = ceil(ceil(n/2)/64) wave pairs, KP = ceil(n/2) rounded up to FK rows,
-// LDF = 128 NH columns (even outputs, then odd outputs).
-static inline int64_t fold_nh(int64_t n) { return cdiv64(cdiv64(n, 2), 64); }
+// Strip sine matrix of length n <= 256: ne = ceil(ceil(n/2)/16) even strips, ceil(floor(n/2)/16) odd strips,
+// max(their sum, 2) waves per workgroup, KP = ceil(n/2) rounded up to FK rows, 16 columns per wave.
 static inline int64_t fold_kp(int64_t n) { return FK * cdiv64(cdiv64(n, 2), FK); }
-static inline int64_t fold_ld(int64_t n) { return 128 * fold_nh(n); }
-// Words of the stored sine matrix of length n (folded for n <= 256, plain otherwise).
-static inline int64_t sine_words(int64_t n) { return n <= 256 ? fold_kp(n) * fold_ld(n) : n * n; }
+static inline int64_t strip_ne(int64_t n) { return cdiv64(cdiv64(n, 2), SW); }
+static inline int64_t strip_count(int64_t n) { return strip_ne(n) + cdiv64(n / 2, SW); }
+static inline int64_t strip_waves(int64_t n) { return strip_count(n) > 2 ? strip_count(n) : 2; }
+static inline int64_t strip_ld(int64_t n) { return SW * strip_waves(n); }
+// slab elements (64 x FK) per thread of a workgroup of strip_waves(n) waves, rounded up to a power of two
+static inline int strip_per(int64_t n) {
+  const int64_t need = cdiv64(TM * FK, 64 * strip_waves(n));
+  return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
+}
+// Words of the stored sine matrix of length n (strip layout for n <= 256, plain otherwise).
+static inline int64_t sine_words(int64_t n) { return n <= 256 ? fold_kp(n) * strip_ld(n) : n * n; }
 
 size_t seqvp_dst_workspace_bytes(int64_t mx, int64_t my, int64_t mz, int64_t ncomp) {
   return (size_t)(ncomp * mx * my * mz + sine_words(mx) + sine_words(my) + sine_words(mz)) * sizeof(double);
@@ -523,40 +517,66 @@ extern "C" int seqvp_mfma_probe(const double* a, const double* b, int fp32, doub
 template <typename T, typename TIN, typename TOUT, int PRE, int POST>
 static void rows_pass(const TIN* X, const double* pre, const T* S, int64_t O, int64_t n, const double* post,
                       double sign, TOUT* Y, hipStream_t s) {
-  const dim3 g(cdiv64(O, TM));
-  const int64_t ldf = fold_ld(n);
-  if (n > 256)
+  if (n > 256) {
     hipLaunchKernelGGL((dst_rows_kernel<T, TIN, TOUT, PRE, POST>), dim3(cdiv64(O, TM) * cdiv64(n, TN)), dim3(256), 0, s,
                        X, pre, S, O, n, post, sign, Y);
-  else if (fold_nh(n) == 1)
-    hipLaunchKernelGGL((dst_rows_fold_kernel<T, TIN, TOUT, PRE, POST, 1>), g, dim3(128), 0, s, X, pre, S, ldf, O, n, post, sign, Y);
-  else
-    hipLaunchKernelGGL((dst_rows_fold_kernel<T, TIN, TOUT, PRE, POST, 2>), g, dim3(256), 0, s, X, pre, S, ldf, O, n, post, sign, Y);
+    return;
+  }
+  const dim3 g(cdiv64(O, TM)), b(64 * strip_waves(n));
+  const int64_t ld = strip_ld(n), ne = strip_ne(n), ns = strip_count(n);
+  switch (strip_per(n)) {
+    case 1:
+      hipLaunchKernelGGL((dst_rows_strip_kernel<T, TIN, TOUT, PRE, POST, 1>), g, b, 0, s, X, pre, S, ld, ne, ns, O, n,
+                         post, sign, Y);
+      break;
+    case 2:
+      hipLaunchKernelGGL((dst_rows_strip_kernel<T, TIN, TOUT, PRE, POST, 2>), g, b, 0, s, X, pre, S, ld, ne, ns, O, n,
+                         post, sign, Y);
+      break;
+    case 4:
+      hipLaunchKernelGGL((dst_rows_strip_kernel<T, TIN, TOUT, PRE, POST, 4>), g, b, 0, s, X, pre, S, ld, ne, ns, O, n,
+                         post, sign, Y);
+      break;
+    default:
+      hipLaunchKernelGGL((dst_rows_strip_kernel<T, TIN, TOUT, PRE, POST, 8>), g, b, 0, s, X, pre, S, ld, ne, ns, O, n,
+                         post, sign, Y);
+  }
 }
 
 // One strided pass of length n over `batch` blocks of inner size I.
 template <typename T, int POST>
 static void cols_pass(const T* X, const T* S, int64_t n, int64_t I, int64_t batch, const double* den, T* Y,
                       hipStream_t s) {
-  const dim3 g(cdiv64(I, TN) * batch);
-  const int64_t ldf = fold_ld(n);
-  if (n > 256)
+  if (n > 256) {
     hipLaunchKernelGGL((dst_cols_kernel<T, POST>), dim3(cdiv64(I, TN) * cdiv64(n, TM) * batch), dim3(256), 0, s, X, S,
                        n, I, den, Y);
-  else if (fold_nh(n) == 1)
-    hipLaunchKernelGGL((dst_cols_fold_kernel<T, POST, 1>), g, dim3(128), 0, s, X, S, ldf, n, I, den, Y);
-  else
-    hipLaunchKernelGGL((dst_cols_fold_kernel<T, POST, 2>), g, dim3(256), 0, s, X, S, ldf, n, I, den, Y);
+    return;
+  }
+  const dim3 g(cdiv64(I, TN) * batch), b(64 * strip_waves(n));
+  const int64_t ld = strip_ld(n), ne = strip_ne(n), ns = strip_count(n);
+  switch (strip_per(n)) {
+    case 1:
+      hipLaunchKernelGGL((dst_cols_strip_kernel<T, POST, 1>), g, b, 0, s, X, S, ld, ne, ns, n, I, den, Y);
+      break;
+    case 2:
+      hipLaunchKernelGGL((dst_cols_strip_kernel<T, POST, 2>), g, b, 0, s, X, S, ld, ne, ns, n, I, den, Y);
+      break;
+    case 4:
+      hipLaunchKernelGGL((dst_cols_strip_kernel<T, POST, 4>), g, b, 0, s, X, S, ld, ne, ns, n, I, den, Y);
+      break;
+    default:
+      hipLaunchKernelGGL((dst_cols_strip_kernel<T, POST, 8>), g, b, 0, s, X, S, ld, ne, ns, n, I, den, Y);
+  }
 }
 
-// The sine matrix of length n into S: folded (fold_kp(n) x fold_ld(n)) for n <= 256, plain n x n otherwise.
+// The sine matrix of length n into S: the strip layout (fold_kp(n) x strip_ld(n)) for n <= 256, plain n x n otherwise.
 template <typename T>
 static void sine_matrix(T* S, int64_t n, hipStream_t s) {
   if (n > 256) {
     hipLaunchKernelGGL((sine_matrix_kernel<T>), dim3(cdiv64(n * n, 256)), dim3(256), 0, s, S, n, n);
   } else {
-    const int64_t kp = fold_kp(n), ld = fold_ld(n);
-    hipLaunchKernelGGL((fold_sine_kernel<T>), dim3(cdiv64(kp * ld, 256)), dim3(256), 0, s, S, n, kp, ld);
+    const int64_t kp = fold_kp(n), ld = strip_ld(n);
+    hipLaunchKernelGGL((strip_sine_kernel<T>), dim3(cdiv64(kp * ld, 256)), dim3(256), 0, s, S, n, kp, ld, strip_ne(n));
   }
 }
 

@@ -87,7 +87,7 @@ def parse():
                          "launches (batch.BatchGroup, launch shapes tuned online)")
     ap.add_argument("--max-wait", type=float, default=.002,
                     help="--batched: seconds the first pair of a launch waits for the others")
-    ap.add_argument("--graphs", choices=("on", "off"), default="on",
+    ap.add_argument("--graphs", choices=("on", "off"), default="off",
                     help="trial and direction segments as captured HIP graphs (on) or issued call by call (off)")
     ap.add_argument("--start", choices=("zero", "half"), default="zero",
                     help="initial map: u = 0 (the configuration's start) or half the known displacement")

@@ -45,7 +45,7 @@ def key_of(*items):
 class SegmentGraphs:
     """Captured graphs of one pair's segments, by (segment name, address key)."""
 
-    def __init__(self, device, capacity=32):
+    def __init__(self, device, capacity=256):
         self.device = torch.device(device)
         self.capacity = int(capacity)
         self._graphs = OrderedDict()

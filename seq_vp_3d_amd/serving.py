@@ -91,7 +91,7 @@ def _download(t):
     return host
 
 
-def run_request(req, device, stream=None, recert_workers=8, batch=None, graphs=True):
+def run_request(req, device, stream=None, recert_workers=8, batch=None, graphs=False):
     """Runs one request on `stream` (the current stream when None): upload, device spline, certified solver from
     u = 0, CPU re-certification of the output. With `batch` (a batch.BatchGroup) the trial-energy sweeps go through
     the group's multi-pair launches. `graphs`: the pair's trial and direction segments as captured HIP graphs (only

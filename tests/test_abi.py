@@ -35,6 +35,7 @@ def test_host_only_queries():
     nz, ny, nx = 160, 192, 224
     ncell = (nz - 1) * (ny - 1) * (nx - 1)
     assert _lib.workspace_bytes(2, (nz, ny, nx)) == 24 * 8 * ncell
-    # the sine matrices are stored folded: ceil(n/2) rows rounded up to 16, 128 * ceil(ceil(n/2)/64) columns
-    assert _lib.workspace_bytes(6, (nz, ny, nx)) == 8 * (3 * 158 * 190 * 222 + 112 * 256 + 96 * 256 + 80 * 256)
+    # the sine matrices are stored folded in 16-output strips: ceil(n/2) rows rounded up to 16, 16 columns per strip
+    # (ceil(ceil(n/2)/16) even strips and ceil(floor(n/2)/16) odd ones)
+    assert _lib.workspace_bytes(6, (nz, ny, nx)) == 8 * (3 * 158 * 190 * 222 + 112 * 224 + 96 * 192 + 80 * 160)
     assert _lib.workspace_bytes(0, (1, 5, 5)) == 0

@@ -293,8 +293,10 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
 // the same per-cell order and the same blocks, so their partials are the same bits): the vertex loads and the edges
 // of the trial and its base are shared, and the closed-point pass runs on edges already in registers.
 // One 256-cell block `blk` of the sweep (the single-pair kernel and the batched persistent kernel share it); er_lds
-// is the workgroup's 36 * BLOCK doubles of LDS for the reference field's edges.
-template <bool WITH_BASE>
+// is the workgroup's 36 * BLOCK doubles of LDS for the reference field's edges. BASE_IS_REF: the base is the
+// reference itself (the outer comparison right after a self-reference refresh), so the base's Gauss-point J and
+// curl are the reference's, already computed: the same values the base's own blends give, bit for bit.
+template <bool WITH_BASE, bool BASE_IS_REF = false>
 __device__ __forceinline__ void trial_energy_block(const int64_t blk, const double* __restrict__ u,
                                                    const double* __restrict__ ref, const double* __restrict__ base,
                                                    const double* __restrict__ k, const Dims& d,
@@ -330,7 +332,7 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
       double Ux[3], Rx[3], Bx[3];
       column_w(Eu, 0, wr, Ux);
       column_col_w(er, BLOCK, 0, wr, Rx);
-      if (WITH_BASE) column_w(Eb, 0, wr, Bx);
+      if (WITH_BASE && !BASE_IS_REF) column_w(Eb, 0, wr, Bx);
 #pragma unroll 1
       for (int p = 3 * row; p < 3 * row + 3; ++p) {
         const double qw = k[K_QW + p];
@@ -346,10 +348,18 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
         cj += qw * dj * dj;
         cc += (qw * d0 * d0 + qw * d1 * d1) + qw * d2 * d2;
         if (WITH_BASE) {
-          double Y[3][3], Ca[3];
-          point_grad_yz_w(Eb, GAUSS_W.w[p], Bx, Y);
-          const double Ja = jacobian(Y);
-          curl_of(Y, Ca);
+          double Ja, Ca[3];
+          if (BASE_IS_REF) {
+            Ja = Jr;
+            Ca[0] = Cr[0];
+            Ca[1] = Cr[1];
+            Ca[2] = Cr[2];
+          } else {
+            double Y[3][3];
+            point_grad_yz_w(Eb, GAUSS_W.w[p], Bx, Y);
+            Ja = jacobian(Y);
+            curl_of(Y, Ca);
+          }
           const double wm = 0.5 * mu * qw, wn = 0.5 * nu * qw;
           double t = wm * ((J - Ja) * (dj + (Ja - Jr)));
 #pragma unroll
@@ -366,7 +376,7 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
   block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
 }
 
-template <bool WITH_BASE>
+template <bool WITH_BASE, bool BASE_IS_REF = false>
 __global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __restrict__ u,
                                                              const double* __restrict__ ref,
                                                              const double* __restrict__ base,
@@ -374,8 +384,8 @@ __global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __
                                                              double* __restrict__ soft_partials,
                                                              double* __restrict__ barrier_partials) {
   __shared__ double er_lds[36 * BLOCK];
-  trial_energy_block<WITH_BASE>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d, soft_partials,
-                                barrier_partials, er_lds);
+  trial_energy_block<WITH_BASE, BASE_IS_REF>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d, soft_partials,
+                                             barrier_partials, er_lds);
 }
 
 // ---- batched trial energies: many pairs in one persistent launch ------------------------------------------------
@@ -1639,7 +1649,9 @@ extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const d
   hipStream_t s = (hipStream_t)stream;
   double* soft = (double*)ws;
   double* bar = soft + nb * 6;
-  if (base)
+  if (base == ref)      // the base is the reference: its Gauss-point terms are the reference's
+    hipLaunchKernelGGL((trial_energy_kernel<true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
+  else if (base)
     hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar);
   else
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);

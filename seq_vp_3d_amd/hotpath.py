@@ -452,7 +452,7 @@ class PairState:
     def snapshot(self):
         """Device copies of the accepted state (the outer start)."""
         return dict(theta=self.theta.clone(), u=self.u.clone(), warped=self.warped.clone(), bgrad=self.bgrad.clone(),
-                    energy=self.energy, certificate=self.certificate)
+                    energy=self.energy, certificate=self.certificate, version=self._version)
 
     def restore(self, snap):
         self._version += 1
@@ -469,7 +469,10 @@ class PairState:
         q, w = self.q1, self.problem.weights
         node = q.node_energy(self.warped, self.fixed, self.u, self.ref, w.lam, data_weight=self.data_weight,
                              base_warped=snap["warped"], base_u=snap["u"])
-        sb = q.trial_energy(self.u, self.ref, snap["u"]).tolist()
+        # a snapshot of the state the reference was set to (the outer start after a refresh) is the reference
+        # itself: passed as such, the sweep takes the base's point terms from the reference's
+        same = snap.get("version") is not None and snap["version"] == self._ref_at
+        sb = q.trial_energy(self.u, self.ref, self.ref if same else snap["u"]).tolist()
         n, s_, b = node.tolist(), sb[0:3], sb[3:7]
         change = q.barrier_weight * b[2]
         return n[2] + n[3] + s_[2] + change, n[2] + change, min(b[1], b[3]) - q.delta

@@ -69,6 +69,13 @@ def test_soft_and_barrier_energy(gpu_device, activation):
     for base in (B, None):
         fused = q.trial_energy(U, R, base)
         assert torch.equal(fused, torch.cat((q.soft_energy(U, R, base), q.barrier_energy(U, base)))), base is None
+    # the reference passed as the base (its own buffer): the base's point terms are taken from the reference's, the
+    # same bits as a separate copy of it; against the oracle the difference is the soft energy itself
+    same = q.trial_energy(U, R, R)
+    assert torch.equal(same, q.trial_energy(U, R, R.clone()))
+    jd, curl, diff = m.soft_energy(u, ref, ref)
+    got = same.cpu().numpy()
+    assert abs(got[2] - diff) <= 1e-12 * (jd + curl) and abs(got[2] - (jd + curl)) <= 1e-12 * (jd + curl)
 
 
 def test_energy_nan_propagates_to_min_j(gpu_device):

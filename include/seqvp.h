@@ -281,6 +281,16 @@ int seqvp_restrict(const double* u, int64_t nx, int64_t ny, int64_t nz, double* 
 int seqvp_axpy_inject(const double* theta, const double* p, double tau, int64_t nx, int64_t ny, int64_t nz,
                       double* theta_out, double* u, void* stream);
 
+/* The scalars of a PG direction in one pass over the free vectors (the host products of poisson_gradient.py:955-1011
+ * and the slope/gradient-norm reductions of :1263, 1307-1315): out[0] = g.p, out[1] = g.g; with the previous
+ * accepted state (theta_old, g_old, p_old; all NULL for none) out[4..7] = seqvp_bb2_products' four values and
+ * out[9..10] = seqvp_max_abs_diff(theta, theta_old)'s two; out[8] = seqvp_step_bound(p, u)'s bound. Every value is
+ * the bits of its separate entry point; out has 11 device doubles (2, 3 untouched); workspace op 7. */
+int seqvp_direction_scalars(const double* theta, const double* theta_old, const double* g, const double* g_old,
+                            const double* p_old, const double* p, const double* u, double ox, double oy, double oz,
+                            double hx, double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* out, void* ws,
+                            size_t ws_bytes, void* stream);
+
 /* The same candidate map with the step read from device memory (tau: one double), so a captured trial segment
  * (seqvp_graph_*) takes a new step on every launch; same arithmetic, same bits. Counterpart of the host axpy of
  * poisson_gradient.py:1403-1437 inside the device trial of accelerated.py:1666-1730. */

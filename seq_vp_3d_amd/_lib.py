@@ -59,6 +59,8 @@ SIGNATURES = {
     "seqvp_restrict": [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_axpy_inject": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_axpy_inject_dev": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
+    "seqvp_direction_scalars": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+                                c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_graph_begin": [c_vp],
     "seqvp_graph_end": [c_vp, ctypes.POINTER(c_vp)],
     "seqvp_graph_launch": [c_vp, c_vp],

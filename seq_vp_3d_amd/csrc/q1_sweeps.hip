@@ -18,12 +18,15 @@
 namespace seqvp {
 
 constexpr int BLOCK = 256;
+// Threads of the single-workgroup reductions of block partials: more loads in flight at once (a partial array of a
+// native sweep is 26,454 blocks; 1024 threads fold 26 each).
+constexpr int RB = 1024;
 
 // ------------------------------------------------------------------------------------------------
 // Block and grid reductions of NS double-double sums and NM NaN-propagating minima.
-template <int NS, int NM>
+template <int NS, int NM, int NT = BLOCK>
 __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
-  __shared__ double sh[BLOCK / 64][2 * NS + NM + 1];
+  __shared__ double sh[NT / 64][2 * NS + NM + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NS; ++q) s[q] = dd_wave_sum(s[q]);
@@ -40,14 +43,14 @@ __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       DD t{sh[0][2 * q], sh[0][2 * q + 1]};
-      for (int w = 1; w < BLOCK / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
+      for (int w = 1; w < NT / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
       out[2 * q] = t.hi;
       out[2 * q + 1] = t.lo;
     }
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
       double t = sh[0][2 * NS + q];
-      for (int w = 1; w < BLOCK / 64; ++w) t = nan_min(t, sh[w][2 * NS + q]);
+      for (int w = 1; w < NT / 64; ++w) t = nan_min(t, sh[w][2 * NS + q]);
       out[2 * NS + q] = t;
     }
   }
@@ -65,14 +68,14 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
   for (int q = 0; q < NS; ++q) s[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = __builtin_inf();
-  // Each thread folds blocks t, t + BLOCK, t + 2 BLOCK, ... in that order; eight of them are loaded at a time
+  // Each thread folds blocks t, t + RB, t + 2 RB, ... in that order; eight of them are loaded at a time
   // (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop.
   constexpr int BATCH = 8;
-  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * BLOCK) {
+  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * RB) {
     double v[BATCH][STRIDE];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-      const int64_t b = b0 + i * BLOCK < nblocks ? b0 + i * BLOCK : nblocks - 1;
+      const int64_t b = b0 + i * RB < nblocks ? b0 + i * RB : nblocks - 1;
 #pragma unroll
       for (int q = 0; q < STRIDE; ++q) v[i][q] = partials[b * STRIDE + q];
     }
@@ -80,7 +83,7 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
     // keeps the eight loads ahead of the arithmetic)
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-      const bool in = b0 + i * BLOCK < nblocks;
+      const bool in = b0 + i * RB < nblocks;
 #pragma unroll
       for (int q = 0; q < NS; ++q) {
         DD t = s[q];
@@ -93,7 +96,7 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
     }
   }
   __shared__ double fin[STRIDE];
-  block_reduce_store<NS, NM>(s, m, fin);
+  block_reduce_store<NS, NM, RB>(s, m, fin);
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -104,7 +107,7 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
 }
 
 template <int NS, int NM, bool INTERLEAVE = false>
-__global__ void __launch_bounds__(BLOCK) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
+__global__ void __launch_bounds__(RB) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
                                                                 double* __restrict__ out) {
   reduce_partials_body<NS, NM, INTERLEAVE>(partials, nblocks, out);
 }
@@ -436,7 +439,7 @@ __global__ void __launch_bounds__(BLOCK, 2) trial_energy_batch_kernel(const Tria
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) trial_energy_batch_reduce_kernel(const TrialBatchTable t) {
+__global__ void __launch_bounds__(RB) trial_energy_batch_reduce_kernel(const TrialBatchTable t) {
   const TrialBatchPair& P = t.p[blockIdx.x];
   if (blockIdx.y == 0)
     reduce_partials_body<3, 0>(P.soft_partials, P.nblocks, P.out);
@@ -1402,27 +1405,27 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) jd_root_reduce_kernel(const double* __restrict__ partials, int64_t nblocks,
+__global__ void __launch_bounds__(RB) jd_root_reduce_kernel(const double* __restrict__ partials, int64_t nblocks,
                                                                double* __restrict__ out) {
   double best = __builtin_inf(), invalid = 0.0, bad = 0.0;
   constexpr int BATCH = 8;   // loads of eight blocks in flight at once; the fold order is unchanged
-  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * BLOCK) {
+  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * RB) {
     double v[BATCH][3];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-      const int64_t b = b0 + i * BLOCK < nblocks ? b0 + i * BLOCK : nblocks - 1;
+      const int64_t b = b0 + i * RB < nblocks ? b0 + i * RB : nblocks - 1;
 #pragma unroll
       for (int q = 0; q < 3; ++q) v[i][q] = partials[3 * b + q];
     }
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
-      const bool in = b0 + i * BLOCK < nblocks;   // selected, not branched (keeps the loads ahead)
+      const bool in = b0 + i * RB < nblocks;   // selected, not branched (keeps the loads ahead)
       best = in ? fmin(best, v[i][0]) : best;
       invalid = in ? invalid + v[i][1] : invalid;
       bad = in ? fmax(bad, v[i][2]) : bad;
     }
   }
-  __shared__ double sh[BLOCK / 64][3];
+  __shared__ double sh[RB / 64][3];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   best = wave_min(best);
 #pragma unroll
@@ -1438,7 +1441,7 @@ __global__ void __launch_bounds__(BLOCK) jd_root_reduce_kernel(const double* __r
   __syncthreads();
   if (threadIdx.x == 0) {
     double b0 = sh[0][0], i0 = sh[0][1], f0 = sh[0][2];
-    for (int w = 1; w < BLOCK / 64; ++w) {
+    for (int w = 1; w < RB / 64; ++w) {
       b0 = fmin(b0, sh[w][0]);
       i0 += sh[w][1];
       f0 = fmax(f0, sh[w][2]);
@@ -1635,7 +1638,7 @@ extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const do
     hipLaunchKernelGGL((soft_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
   else
     hipLaunchKernelGGL((soft_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
-  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -1655,8 +1658,8 @@ extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const d
     hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar);
   else
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
-  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, soft, nb, out);
-  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, bar, nb, out + 3);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, soft, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, bar, nb, out + 3);
   return launch_status();
 }
 
@@ -1697,7 +1700,7 @@ extern "C" int seqvp_q1_trial_energy_batch(const SeqvpTrialBatchEntry* pairs, in
   const int64_t g = grid == 0 ? items : std::min<int64_t>(cdiv(grid, 8) * 8, items);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(trial_energy_batch_kernel, dim3(g), dim3(BLOCK), 0, s, t);
-  hipLaunchKernelGGL(trial_energy_batch_reduce_kernel, dim3(npairs, 2), dim3(BLOCK), 0, s, t);
+  hipLaunchKernelGGL(trial_energy_batch_reduce_kernel, dim3(npairs, 2), dim3(RB), 0, s, t);
   return launch_status();
 }
 
@@ -1749,7 +1752,7 @@ extern "C" int seqvp_q1_soft_energy_noalloc(const double* u, const double* ref, 
     hipLaunchKernelGGL((soft_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
   else
     hipLaunchKernelGGL((soft_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
-  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -1766,7 +1769,7 @@ extern "C" int seqvp_q1_barrier_energy(const double* u, const double* base, cons
   else
     hipLaunchKernelGGL((barrier_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, part);
   // partials [val hi, val lo, chg hi, chg lo, minJ, min base J] -> out [val, minJ, chg, min base J]
-  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -1952,7 +1955,7 @@ extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, 
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)ws;
   hipLaunchKernelGGL((jd_root_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, v, k, limit, d, part);
-  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -1965,7 +1968,7 @@ extern "C" int seqvp_jd_root_free(const double* u, const double* p, const double
   hipStream_t s = (hipStream_t)stream;
   double* part = (double*)ws;
   hipLaunchKernelGGL((jd_root_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, p, k, limit, d, part);
-  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -1982,7 +1985,7 @@ extern "C" int seqvp_node_energy(const double* warped, const double* fixed, cons
   double* part = (double*)ws;
   hipLaunchKernelGGL(node_energy_kernel, dim3(nb), dim3(BLOCK), 0, s, warped, fixed, data_weight, u, ref, base_warped,
                      base_u, lam, hx * hy * hz, d, part);
-  hipLaunchKernelGGL((reduce_partials_kernel<4, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<4, 0>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -2013,7 +2016,7 @@ extern "C" int seqvp_q1_soft_energy_f32(const double* u, const double* ref, cons
     hipLaunchKernelGGL((soft_energy_f32_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, part);
   else
     hipLaunchKernelGGL((soft_energy_f32_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, part);
-  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }
 
@@ -2030,6 +2033,6 @@ extern "C" int seqvp_q1_barrier_energy_f32(const double* u, const double* base, 
     hipLaunchKernelGGL((barrier_energy_f32_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, base, k, d, part);
   else
     hipLaunchKernelGGL((barrier_energy_f32_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, u, k, d, part);
-  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(BLOCK), 0, s, part, nb, out);
+  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, part, nb, out);
   return launch_status();
 }

@@ -271,13 +271,148 @@ __global__ void __launch_bounds__(64) fold_min_max_kernel(const double* __restri
   }
 }
 
+// The scalars of a PG direction in one pass over the free vector (n = 3F entries, component-major): g.p and g.g
+// as dots_kernel<2> forms them; with the previous accepted state (th0, g0, p0 not NULL) the BB2 products as
+// bb2_kernel forms them and the last step's max |th - th0|, max |th0| as maxabs_kernel does; and the domain step
+// bound of p at X + u as step_bound_kernel does. Same per-element operations in the same grid-stride order per
+// accumulator, so every value is the bits of its separate kernel. Partials: 16 doubles per block (six
+// double-double sums, the bound, the two maxima).
+__global__ void __launch_bounds__(VB) direction_scalars_kernel(const double* __restrict__ th,
+                                                               const double* __restrict__ th0,
+                                                               const double* __restrict__ g,
+                                                               const double* __restrict__ g0,
+                                                               const double* __restrict__ p0,
+                                                               const double* __restrict__ p,
+                                                               const double* __restrict__ u, double ox, double oy,
+                                                               double oz, double hx, double hy, double hz, int64_t nx,
+                                                               int64_t ny, int64_t nz, double* __restrict__ partials) {
+  const int64_t mx = nx - 2, my = ny - 2, mz = nz - 2, F = mx * my * mz, n = 3 * F;
+  const double lo[3] = {ox, oy, oz};
+  const double hi[3] = {ox + hx * (double)(nx - 1), oy + hy * (double)(ny - 1), oz + hz * (double)(nz - 1)};
+  const double h[3] = {hx, hy, hz};
+  const bool prev = th0 != nullptr;
+  DD acc[6] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double best = __builtin_inf(), m0 = 0.0, m1 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  for (int64_t i = (int64_t)blockIdx.x * VB + threadIdx.x; i < n; i += stride) {
+    const double gi = g[i], pi = p[i];
+    const double dp[2][2] = {{gi, pi}, {gi, gi}};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const double prod = dp[q][0] * dp[q][1];
+      const double err = fma(dp[q][0], dp[q][1], -prod);
+      dd_add(acc[q], prod);
+      acc[q].lo += err;
+    }
+    if (prev) {
+      const double t0 = th0[i];
+      const double sv = th[i] - t0, yv = gi - g0[i], iy = p0[i] - pi;
+      const double pr[4][2] = {{sv, yv}, {sv, sv}, {yv, yv}, {yv, iy}};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double prod = pr[q][0] * pr[q][1];
+        const double err = fma(pr[q][0], pr[q][1], -prod);
+        dd_add(acc[2 + q], prod);
+        acc[2 + q].lo += err;
+      }
+      m0 = (fabs(sv) > m0 || sv != sv) ? fabs(sv) : m0;   // NaN propagates
+      m1 = (fabs(t0) > m1 || t0 != t0) ? fabs(t0) : m1;
+    }
+    const int64_t c = i / F, f = i - c * F;
+    const int64_t x = f % mx + 1, y = (f / mx) % my + 1, z = f / (mx * my) + 1;
+    const int64_t idx = c == 0 ? x : (c == 1 ? y : z);
+    const double xc = (lo[c] + h[c] * (double)idx) + u[3 * ((z * ny + y) * nx + x) + c];
+    if (pi > 0.0) best = fmin(best, (hi[c] - xc) / pi);
+    else if (pi < 0.0) best = fmin(best, (lo[c] - xc) / pi);
+  }
+  __shared__ double sh[VB / 64][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) acc[q] = dd_wave_sum(acc[q]);
+  best = wave_min(best);
+  m0 = wave_max(m0);
+  m1 = wave_max(m1);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      sh[wave][2 * q] = acc[q].hi;
+      sh[wave][2 * q + 1] = acc[q].lo;
+    }
+    sh[wave][12] = best;
+    sh[wave][13] = m0;
+    sh[wave][14] = m1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double* o = partials + (int64_t)blockIdx.x * 16;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      DD t{sh[0][2 * q], sh[0][2 * q + 1]};
+      for (int w = 1; w < VB / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
+      o[2 * q] = t.hi;
+      o[2 * q + 1] = t.lo;
+    }
+    o[12] = fmin(fmin(sh[0][12], sh[1][12]), fmin(sh[2][12], sh[3][12]));
+    o[13] = fmax(fmax(sh[0][13], sh[1][13]), fmax(sh[2][13], sh[3][13]));
+    o[14] = fmax(fmax(sh[0][14], sh[1][14]), fmax(sh[2][14], sh[3][14]));
+    o[15] = 0.0;
+  }
+}
+
+// out[0] g.p, out[1] g.g, out[4..7] the BB2 products, out[8] the step bound, out[9..10] the step maxima (each sum
+// folded as dots_reduce_kernel folds it, one wave per sum; the bound and maxima as fold_min_max_kernel folds them).
+__global__ void __launch_bounds__(512) direction_scalars_reduce_kernel(const double* __restrict__ partials, int nblocks,
+                                                                       int prev, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  if (q < 6) {
+    if (q >= 2 && !prev) return;
+    DD s{0.0, 0.0};
+    for (int b = lane; b < nblocks; b += 64) dd_add(s, DD{partials[b * 16 + 2 * q], partials[b * 16 + 2 * q + 1]});
+    s = dd_wave_sum(s);
+    if (lane == 0) out[q < 2 ? q : q + 2] = s.hi + s.lo;
+  } else if (q == 6) {
+    double m = __builtin_inf();
+    for (int b = lane; b < nblocks; b += 64) m = fmin(m, partials[b * 16 + 12]);
+    m = wave_min(m);
+    if (lane == 0) out[8] = m;
+  } else if (prev) {
+    double a = 0.0, c = 0.0;
+    for (int b = lane; b < nblocks; b += 64) {
+      a = fmax(a, partials[b * 16 + 13]);
+      c = fmax(c, partials[b * 16 + 14]);
+    }
+    a = wave_max(a);
+    c = wave_max(c);
+    if (lane == 0) {
+      out[9] = a;
+      out[10] = c;
+    }
+  }
+}
+
 }  // namespace seqvp
 
 using namespace seqvp;
 
 size_t seqvp_vec_workspace_bytes(int64_t n) {
   (void)n;
-  return (size_t)VGRID * 8 * sizeof(double);
+  return (size_t)VGRID * 16 * sizeof(double);
+}
+
+extern "C" int seqvp_direction_scalars(const double* theta, const double* theta_old, const double* g,
+                                       const double* g_old, const double* p_old, const double* p, const double* u,
+                                       double ox, double oy, double oz, double hx, double hy, double hz, int64_t nx,
+                                       int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+  if (!theta || !g || !p || !u || !out || nx < 3 || ny < 3 || nz < 3) return SEQVP_EINVAL;
+  const bool prev = theta_old != nullptr;
+  if (prev && (!g_old || !p_old)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < seqvp_vec_workspace_bytes(0)) return SEQVP_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = (double*)ws;
+  hipLaunchKernelGGL(direction_scalars_kernel, dim3(VGRID), dim3(VB), 0, s, theta, theta_old, g, g_old, p_old, p, u,
+                     ox, oy, oz, hx, hy, hz, nx, ny, nz, part);
+  hipLaunchKernelGGL(direction_scalars_reduce_kernel, dim3(1), dim3(512), 0, s, part, VGRID, prev ? 1 : 0, out);
+  return launch_status();
 }
 
 extern "C" int seqvp_dots(const double* const* xs, const double* const* ys, int npairs, int64_t n, double* out,

@@ -123,6 +123,7 @@ class PairState:
         # synchronisation
         self.scalars = z(12)
         self._prev = None           # (theta, g, p) of the last accepted state, kept for the next BB2 products
+        self._ratio_pair = None     # (new theta, old theta) of the last accepted step, its ratio not computed yet
         # The accepted state's version (bumped whenever it changes), the version at which the reference was set to
         # it, and the version whose outer gradient (image + barrier) the joint pass left in diag.
         self._version, self._ref_at, self._gf_at = 0, None, None
@@ -519,42 +520,40 @@ class PairState:
         gradient before it (gradient()); all of it runs as one segment."""
         sweep = gradient and not self._take_outer_gradient()
         prev_key = None if prev is None else (prev["theta"], prev["g"], prev["p"])
-        key = key_of("direction", sweep, fp32, norm, *(prev_key or (None,)), self.g, self.p, self.u, self.theta,
-                     self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale, self._den,
-                     self.q1.ws.buf, self.dot_ws, self.scalars)
-        self._segment(key, lambda: self._direction_segment(prev, norm, fp32, sweep))
+        # the pending step ratio comes with the one-pass scalars when its pair is (theta, the history's theta), the
+        # pair promote leaves; any other pair gets its own pass inside the segment
+        pair = self._ratio_pair
+        extra = None if pair is None or (prev is not None and pair[0] is self.theta and pair[1] is prev["theta"]) \
+            else pair
+        self._ratio_pair = None
+        key = key_of("direction", sweep, fp32, *(prev_key or (None,)), *(extra or (None,)), self.g, self.p,
+                     self.u, self.theta, self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale,
+                     self._den, self.q1.ws.buf, self.dot_ws, self.scalars)
+        self._segment(key, lambda: self._direction_segment(prev, fp32, sweep, extra))
         self._wait()
         v = self._scalars_np.tolist()
         bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
         G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None
         return v[0], bb2, v[8], G, (self._ratio(v[9], v[10]) if ratio else None)
 
-    def _direction_segment(self, prev, norm, fp32, sweep):
+    def _direction_segment(self, prev, fp32, sweep, extra):
         if sweep:
             self._gradient_sweep()
         if fp32 is not None:
             self.direction(fp32=fp32)
+        # g.p, g.g, the BB2 products with the previous accepted state, the step bound and the last step's maxima
+        # in one pass (each value the bits of its separate entry point); the step ratio's maxima are those of
+        # (theta, prev theta), the pair step_ratio_launch recorded
         out = self.scalars
-        g, p = self.g, self.p
-        xs, ys = (ctypes.c_void_p * 1)(g.data_ptr()), (ctypes.c_void_p * 1)(p.data_ptr())
-        if norm:
-            # g.p and g.g in one pass (one launch, g read once from HBM)
-            xs2 = (ctypes.c_void_p * 2)(g.data_ptr(), g.data_ptr())
-            ys2 = (ctypes.c_void_p * 2)(p.data_ptr(), g.data_ptr())
-            _lib.call("seqvp_dots", xs2, ys2, 2, g.numel(), _lib.ptr(out[0:2]), _lib.ptr(self.dot_ws),
-                      self.dot_ws.numel(), self._stream())
-        else:
-            _lib.call("seqvp_dots", xs, ys, 1, g.numel(), _lib.ptr(out[0:1]), _lib.ptr(self.dot_ws),
-                      self.dot_ws.numel(), self._stream())
-        if prev is not None:
-            _lib.call("seqvp_bb2_products", _lib.ptr(self.theta), _lib.ptr(prev["theta"]), _lib.ptr(self.g),
-                      _lib.ptr(prev["g"]), _lib.ptr(prev["p"]), _lib.ptr(self.p), self.nfree, _lib.ptr(out[4:8]),
-                      _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
         nz, ny, nx = self.shape
         hx, hy, hz = self.grid.spacing
         ox, oy, oz = self.grid.origin
-        _lib.call("seqvp_step_bound", _lib.ptr(self.p), _lib.ptr(self.u), ox, oy, oz, hx, hy, hz, nx, ny, nz,
-                  _lib.ptr(out[8:9]), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        pv = (None, None, None) if prev is None else (prev["theta"], prev["g"], prev["p"])
+        _lib.call("seqvp_direction_scalars", _lib.ptr(self.theta), _lib.ptr(pv[0]), _lib.ptr(self.g), _lib.ptr(pv[1]),
+                  _lib.ptr(pv[2]), _lib.ptr(self.p), _lib.ptr(self.u), ox, oy, oz, hx, hy, hz, nx, ny, nz,
+                  _lib.ptr(out), _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
+        if extra is not None:
+            self._launch_step_ratio(*extra)
         _lib.call("seqvp_copy_async", ctypes.c_void_p(self._scalars_host.data_ptr()), _lib.ptr(out), 12 * 8,
                   self._stream())
 
@@ -578,11 +577,20 @@ class PairState:
         return float(np.float64(top) / max(length, bmax))
 
     def step_ratio_launch(self, a, b):
-        """step_ratio(a, b) left on the device, read by the next direction_scalars(ratio=True) or read_step_ratio."""
+        """step_ratio(a, b), read by the next direction_scalars(ratio=True) or read_step_ratio. Called with the
+        accepted trial and the state it replaces, right before promote: the pair is recorded, and the next
+        direction's one-pass scalars compute its maxima (a and b are then theta and the history's theta); a
+        read_step_ratio before that launches them on its own."""
+        self._ratio_pair = (a, b)
+
+    def _launch_step_ratio(self, a, b):
         _lib.call("seqvp_max_abs_diff", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.scalars[9:11]),
                   _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
 
     def read_step_ratio(self):
+        if self._ratio_pair is not None:
+            self._launch_step_ratio(*self._ratio_pair)
+            self._ratio_pair = None
         top, bmax = self.scalars[9:11].tolist()
         return self._ratio(top, bmax)
 

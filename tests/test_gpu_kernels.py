@@ -315,6 +315,55 @@ def test_vector_ops(gpu_device):
     np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-15)
 
 
+@pytest.mark.parametrize("shape", [(9, 10, 11), (40, 33, 57)])
+def test_direction_scalars_are_the_separate_entry_points(gpu_device, shape):
+    """seqvp_direction_scalars (one pass) against seqvp_dots, seqvp_bb2_products, seqvp_step_bound and
+    seqvp_max_abs_diff, bit for bit, with and without the previous state, with a NaN maximum case too."""
+    import ctypes
+    from seq_vp_3d_amd import _lib
+    print("case direction scalars", shape, flush=True)
+    nz, ny, nx = shape
+    og = Q.Grid(shape, (1., 1.3, .8))
+    rng = np.random.default_rng(98)
+    n = 3 * og.n_free_scalar
+    th, th0, g, g0, p0, p = (dev(rng.normal(size=n), gpu_device) for _ in range(6))
+    u = dev(rng.normal(size=shape + (3,)) * .1, gpu_device)
+    ws = torch.empty(_lib.workspace_bytes(7, shape), dtype=torch.uint8, device=gpu_device)
+    s = _lib.stream_handle(gpu_device)
+    ox, oy, oz = 0., 0., 0.
+    hx, hy, hz = 1., 1.3, .8
+
+    def separate(prev):
+        out = torch.full((12,), np.nan, dtype=torch.float64, device=gpu_device)
+        xs = (ctypes.c_void_p * 2)(g.data_ptr(), g.data_ptr())
+        ys = (ctypes.c_void_p * 2)(p.data_ptr(), g.data_ptr())
+        _lib.call("seqvp_dots", xs, ys, 2, n, _lib.ptr(out[0:2]), _lib.ptr(ws), ws.numel(), s)
+        if prev:
+            _lib.call("seqvp_bb2_products", _lib.ptr(th), _lib.ptr(th0), _lib.ptr(g), _lib.ptr(g0), _lib.ptr(p0),
+                      _lib.ptr(p), n, _lib.ptr(out[4:8]), _lib.ptr(ws), ws.numel(), s)
+            _lib.call("seqvp_max_abs_diff", _lib.ptr(th), _lib.ptr(th0), n, _lib.ptr(out[9:11]), _lib.ptr(ws),
+                      ws.numel(), s)
+        _lib.call("seqvp_step_bound", _lib.ptr(p), _lib.ptr(u), ox, oy, oz, hx, hy, hz, nx, ny, nz, _lib.ptr(out[8:9]),
+                  _lib.ptr(ws), ws.numel(), s)
+        return out.cpu().numpy()
+
+    def fused(prev):
+        out = torch.full((12,), np.nan, dtype=torch.float64, device=gpu_device)
+        a = (th0, g0, p0) if prev else (None, None, None)
+        _lib.call("seqvp_direction_scalars", _lib.ptr(th), _lib.ptr(a[0]), _lib.ptr(g), _lib.ptr(a[1]), _lib.ptr(a[2]),
+                  _lib.ptr(p), _lib.ptr(u), ox, oy, oz, hx, hy, hz, nx, ny, nz, _lib.ptr(out), _lib.ptr(ws),
+                  ws.numel(), s)
+        return out.cpu().numpy()
+
+    for prev in (True, False):
+        a, b = separate(prev), fused(prev)
+        assert np.array_equal(a, b, equal_nan=True), (prev, a, b)
+        assert np.isfinite(b[8]) and (np.isfinite(b[4:8]).all() if prev else np.isnan(b[4:8]).all())
+    th0[7] = float("nan")
+    a, b = separate(True), fused(True)
+    assert np.array_equal(a, b, equal_nan=True)
+
+
 def test_node_energy(gpu_device):
     from seq_vp_3d_amd.q1 import FusedQ1
     prob, og, m = case()

@@ -183,3 +183,33 @@ def test_native_gradient_is_the_derivative_of_the_energy(native):
     d_bar = q.barrier_energy(up, um).tolist()[2] * q.barrier_weight
     fd = (d_soft + d_bar) / (2 * h)
     assert abs(fd - slope) <= 1e-6 * abs(slope), (fd, slope)
+
+
+def test_native_pair_in_a_batched_trial_launch(native):
+    """The native fields in one multi-pair persistent launch with two uneven pairs (96^3 and 128x96x80): the native
+    pair's 7 trial results equal its single-pair sweep bit for bit, for a persistent and a one-item-per-workgroup
+    launch shape."""
+    from seq_vp_3d_amd import _lib, batch as B
+    from seq_vp_3d_amd.problem import Q1Problem, make_grid_3d
+    from seq_vp_3d_amd.q1 import OP_TRIAL, FusedQ1
+    print("case native batched trial launch", flush=True)
+    q = native[.05]
+    dev = native["dev"]
+    items = [(q, native["U"], native["R"], native["B"])]
+    for i, shape in enumerate([(96, 96, 96), (80, 96, 128)]):
+        nz, ny, nx = shape
+        g = make_grid_3d(nx=nx, ny=ny, nz=nz, hx=1., hy=1., hz=1.)
+        qs = FusedQ1(Q1Problem(g, native["w"], 0., .05), q.device)
+        r = smooth_field(shape, 2., 10 + i)
+        b = r + smooth_field(shape, 1., 20 + i)
+        items.append((qs, dev(b + rough_field(shape, .01, 30 + i)), dev(r), dev(b)))
+    want = [qq.trial_energy(u, r, b) for qq, u, r, b in items]
+    handle = _lib.stream_handle(q.device)
+    for tile, grid in ((1, 0), (2, 1024)):
+        outs = [torch.full((7,), float("nan"), dtype=torch.float64, device=q.device) for _ in items]
+        wss = [torch.empty(_lib.workspace_bytes(OP_TRIAL, qq.shape), dtype=torch.uint8, device=q.device)
+               for qq, *_ in items]
+        B.trial_energy_batch([B.entry(qq, u, r, b, o, ws) for (qq, u, r, b), o, ws in zip(items, outs, wss)], tile, grid,
+                             handle)
+        for o, w_ in zip(outs, want):
+            assert torch.equal(o, w_), (tile, grid)

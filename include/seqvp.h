@@ -53,6 +53,16 @@ int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base
                           int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                           size_t workspace_bytes, void* stream);
 
+/* seqvp_q1_trial_energy for a base whose least closed27 J is known (the same trial energy call,
+ * triton_q1.py:572-609, given what the previous evaluation of that state returned as its min J): `base_min` is one
+ * device double holding the base's least closed27 J over the grid, which the caller has checked is at or above the
+ * barrier activation k[37], so that no base point is active. The sweep then evaluates no base closed27 J; out[6] is
+ * *base_min and every other output is seqvp_q1_trial_energy's bit for bit. If *base_min is below the activation,
+ * out[3] is NaN. base_min == NULL, base == NULL or base == ref: exactly seqvp_q1_trial_energy. */
+int seqvp_q1_trial_energy_base_min(const double* u, const double* ref, const double* base, const double* base_min,
+                                   const double* k, int64_t nx, int64_t ny, int64_t nz, double* out,
+                                   void* workspace, size_t workspace_bytes, void* stream);
+
 /* One pair of a batched trial-energy launch: the arguments of seqvp_q1_trial_energy (base required). */
 typedef struct {
   const double* u;

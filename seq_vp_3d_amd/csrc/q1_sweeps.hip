@@ -208,31 +208,49 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
 }
 
 // Closed27 barrier terms of one cell from its edges (and its base's): the minima of J (and base J) over the 27 points
-// into m, the barrier value into s[0] and, against a base, the unweighted stable change into s[1].
-template <bool WITH_BASE>
-__device__ __forceinline__ void barrier_cell(Edges& Eu, Edges& Eb, const double* __restrict__ k, DD s[2], double m[2]) {
+// into m, the barrier value into s[0] and, against a base, the unweighted stable change into s[1]. QUIET: the
+// caller knows the base's least closed27 J over the whole grid, base_min, and that it is >= act, so no base point is
+// active and the first pass needs no base J: the wave's activity is its own points', and the base minimum is that
+// value (the same bits: the base's J are the same function of the same edges wherever they are computed).
+template <bool WITH_BASE, bool QUIET = false>
+__device__ __forceinline__ void barrier_cell(Edges& Eu, Edges& Eb, const double* __restrict__ k, DD s[2], double m[2],
+                                             double base_min = 0.0) {
+  static_assert(!QUIET || WITH_BASE, "a quiet base needs a base");
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
   // Minima over the 27 points, each with its compile-time closed blend, and whether any point is active.
   bool active = false;
-  static_for<27>([&](auto P) {
-    if constexpr (P % 9 == 0) {
-      opaque_edges(Eu);
-      if (WITH_BASE) opaque_edges(Eb);
-    }
-    double V[3][3];
-    closed_point_grad<P>(Eu, V);
-    const double J = jacobian(V);
-    m[0] = nan_min(m[0], J);
-    if (WITH_BASE) {
-      double Y[3][3];
-      closed_point_grad<P>(Eb, Y);
-      const double Ja = jacobian(Y);
-      m[1] = nan_min(m[1], Ja);
-      active |= !(J >= act && Ja >= act);
-    } else {
-      active |= J < act;
-    }
-  });
+  if constexpr (QUIET) {
+    static_for<27>([&](auto P) {
+      if constexpr (P % 9 == 0) opaque_edges(Eu);
+      double V[3][3];
+      closed_point_grad<P>(Eu, V);
+      const double J = jacobian(V);
+      m[0] = nan_min(m[0], J);
+      active |= !(J >= act);
+    });
+    m[1] = base_min;
+    if (!(base_min >= act)) dd_add(s[0], __builtin_nan(""));   // a broken guarantee gives a NaN barrier value
+  } else {
+    static_for<27>([&](auto P) {
+      if constexpr (P % 9 == 0) {
+        opaque_edges(Eu);
+        if (WITH_BASE) opaque_edges(Eb);
+      }
+      double V[3][3];
+      closed_point_grad<P>(Eu, V);
+      const double J = jacobian(V);
+      m[0] = nan_min(m[0], J);
+      if (WITH_BASE) {
+        double Y[3][3];
+        closed_point_grad<P>(Eb, Y);
+        const double Ja = jacobian(Y);
+        m[1] = nan_min(m[1], Ja);
+        active |= !(J >= act && Ja >= act);
+      } else {
+        active |= J < act;
+      }
+    });
+  }
   // The barrier terms are zero at every point with J >= act (and, against a base, Ja >= act), so only a wave
   // with an active point evaluates them, over the points in order with the tabulated blends.
   if (__any(active)) {
@@ -299,12 +317,13 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
 // is the workgroup's 36 * BLOCK doubles of LDS for the reference field's edges. BASE_IS_REF: the base is the
 // reference itself (the outer comparison right after a self-reference refresh), so the base's Gauss-point J and
 // curl are the reference's, already computed: the same values the base's own blends give, bit for bit.
-template <bool WITH_BASE, bool BASE_IS_REF = false>
+template <bool WITH_BASE, bool BASE_IS_REF = false, bool QUIET = false>
 __device__ __forceinline__ void trial_energy_block(const int64_t blk, const double* __restrict__ u,
                                                    const double* __restrict__ ref, const double* __restrict__ base,
                                                    const double* __restrict__ k, const Dims& d,
                                                    double* __restrict__ soft_partials,
-                                                   double* __restrict__ barrier_partials, double* er_lds) {
+                                                   double* __restrict__ barrier_partials, double* er_lds,
+                                                   double base_min = 0.0) {
   const int64_t c = blk * BLOCK + threadIdx.x;
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   DD sb[2] = {{0.0, 0.0}, {0.0, 0.0}};
@@ -373,22 +392,24 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
     }
     dd_add(s[0], 0.5 * mu * cj);
     dd_add(s[1], 0.5 * nu * cc);
-    barrier_cell<WITH_BASE>(Eu, Eb, k, sb, m);
+    barrier_cell<WITH_BASE, QUIET>(Eu, Eb, k, sb, m, base_min);
   }
   block_reduce_store<3, 0>(s, none, soft_partials + blk * 6);
   block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
 }
 
-template <bool WITH_BASE, bool BASE_IS_REF = false>
+template <bool WITH_BASE, bool BASE_IS_REF = false, bool QUIET = false>
 __global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __restrict__ u,
                                                              const double* __restrict__ ref,
                                                              const double* __restrict__ base,
                                                              const double* __restrict__ k, Dims d,
                                                              double* __restrict__ soft_partials,
-                                                             double* __restrict__ barrier_partials) {
+                                                             double* __restrict__ barrier_partials,
+                                                             const double* __restrict__ base_min) {
   __shared__ double er_lds[36 * BLOCK];
-  trial_energy_block<WITH_BASE, BASE_IS_REF>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d, soft_partials,
-                                             barrier_partials, er_lds);
+  trial_energy_block<WITH_BASE, BASE_IS_REF, QUIET>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d,
+                                                    soft_partials, barrier_partials, er_lds,
+                                                    QUIET ? *base_min : 0.0);
 }
 
 // ---- batched trial energies: many pairs in one persistent launch ------------------------------------------------
@@ -1642,9 +1663,9 @@ extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const do
   return launch_status();
 }
 
-extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
-                                     int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
-                                     void* stream) {
+extern "C" int seqvp_q1_trial_energy_base_min(const double* u, const double* ref, const double* base,
+                                              const double* base_min, const double* k, int64_t nx, int64_t ny,
+                                              int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
   if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(10, nx, ny, nz)) return SEQVP_EWORKSPACE;
   const Dims d = make_dims(nx, ny, nz);
@@ -1653,14 +1674,26 @@ extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const d
   double* soft = (double*)ws;
   double* bar = soft + nb * 6;
   if (base == ref)      // the base is the reference: its Gauss-point terms are the reference's
-    hipLaunchKernelGGL((trial_energy_kernel<true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
+    hipLaunchKernelGGL((trial_energy_kernel<true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
+                       (const double*)nullptr);
+  else if (base && base_min)   // a base with no active closed27 point (the caller's guarantee)
+    hipLaunchKernelGGL((trial_energy_kernel<true, false, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d,
+                       soft, bar, base_min);
   else if (base)
-    hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar);
+    hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar,
+                       (const double*)nullptr);
   else
-    hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar);
+    hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
+                       (const double*)nullptr);
   hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, soft, nb, out);
   hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, bar, nb, out + 3);
   return launch_status();
+}
+
+extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
+                                     int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                     void* stream) {
+  return seqvp_q1_trial_energy_base_min(u, ref, base, nullptr, k, nx, ny, nz, out, ws, ws_bytes, stream);
 }
 
 extern "C" int seqvp_q1_trial_energy_batch(const SeqvpTrialBatchEntry* pairs, int npairs, int tile, int64_t grid,

@@ -145,6 +145,9 @@ class PairState:
         self._tau_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
         self._tau_dev = z(1)
         self._trial_dev = z(17)
+        # least closed27 J of the accepted state when known (host float and device copy): at or above the
+        # activation, the trial sweep needs no closed27 J of the accepted state
+        self._base_min, self._base_min_dev, self._trial_min = None, z(1), None
         self._trial_host = torch.zeros(17, dtype=torch.float64, pin_memory=True)
         self._scalars_host = torch.zeros(12, dtype=torch.float64, pin_memory=True)
         self._tau_np, self._trial_np, self._scalars_np = (t.numpy() for t in (self._tau_host, self._trial_host,
@@ -207,6 +210,16 @@ class PairState:
         if int(bad.item()):
             raise FloatingPointError("state outside the sampling support")
         self.energy = self.evaluate_energy(self.u, self.warped)
+        self._set_base_min(self._evaluated_min)
+
+    def _set_base_min(self, value):
+        self._base_min = value
+        if self._quiet():
+            self._base_min_dev.fill_(value)
+
+    def _quiet(self):
+        """Whether the accepted state has no active closed27 point (known from its last evaluation)."""
+        return self._base_min is not None and self._base_min >= self.q1.activation
 
     def state_certificate(self):
         """The certificate of the accepted map: the one its trial or transfer passed, else screened now."""
@@ -221,6 +234,7 @@ class PairState:
         sb = q.trial_energy(u, self.ref)
         n, v = node.tolist(), sb.tolist()
         s, b = v[0:3], v[3:7]
+        self._evaluated_min = b[1]
         if not b[1] > q.delta:
             raise FloatingPointError("device barrier points contradict a feasible state")
         return Energies(n[0], n[1], s[0], s[1], b[0])
@@ -348,7 +362,7 @@ class PairState:
         # needs no second warp at promotion), the energies, and one copy of the 17 scalars to the host
         self._tau_np[0] = tau
         batched = self.batch is not None
-        key = key_of("trial", batched, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
+        key = key_of("trial", batched, self._quiet(), self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
                      self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
                      self.cert.capacity)
         self._segment(key, lambda: self._trial_segment(not batched))
@@ -361,6 +375,7 @@ class PairState:
         self._wait()
         vals = self._trial_np.tolist()
         nbad = int(self._trial_np[11:12].view(np.int64)[0])
+        self._trial_min = vals[8]
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
 
     def _trial_segment(self, energies=True):
@@ -375,7 +390,8 @@ class PairState:
         q.node_energy(self.warped_t, self.fixed, self.u_t, self.ref, w.lam, data_weight=self.data_weight,
                       base_warped=self.warped, base_u=self.u, out=d[0:4])
         if energies:
-            q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11])      # soft and barrier terms in one sweep
+            q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11],      # soft and barrier terms in one sweep
+                           base_min=self._base_min_dev if self._quiet() else None)
             _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
 
     def trial_energies(self, screen=False):
@@ -387,8 +403,10 @@ class PairState:
         if screen:
             sb = torch.cat((q.soft_energy_f32(self.u_t, self.ref, self.u), q.barrier_energy_f32(self.u_t, self.u)))
         else:
-            sb = q.trial_energy(self.u_t, self.ref, self.u)       # soft and barrier terms in one sweep
+            sb = q.trial_energy(self.u_t, self.ref, self.u,       # soft and barrier terms in one sweep
+                                base_min=self._base_min_dev if self._quiet() else None)
         vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
+        self._trial_min = None if screen else vals[8]      # the FP32 screen's min J is not the closed27 J
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)
 
     def _trial_result(self, n, s, b, nbad, summary, screen):
@@ -444,6 +462,7 @@ class PairState:
         self.bgrad, self.bgrad_t = self.bgrad_t, self.bgrad
         self.energy = energy
         self.certificate = self._certificate
+        self._set_base_min(self._trial_min)
         if not keep_history:
             return None
         self.theta_t, self._prev["theta"] = self._prev["theta"], self.theta_t
@@ -453,7 +472,8 @@ class PairState:
     def snapshot(self):
         """Device copies of the accepted state (the outer start)."""
         return dict(theta=self.theta.clone(), u=self.u.clone(), warped=self.warped.clone(), bgrad=self.bgrad.clone(),
-                    energy=self.energy, certificate=self.certificate, version=self._version)
+                    energy=self.energy, certificate=self.certificate, version=self._version,
+                    base_min=self._base_min)
 
     def restore(self, snap):
         self._version += 1
@@ -463,6 +483,7 @@ class PairState:
         self.bgrad.copy_(snap["bgrad"])
         self.energy = snap["energy"]
         self.certificate = snap.get("certificate")
+        self._set_base_min(snap.get("base_min"))
 
     def compare_to(self, snap):
         """(E(current) - E(snap), F(current) - F(snap), min gap): stable differences of the current state

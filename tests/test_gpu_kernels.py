@@ -78,6 +78,29 @@ def test_soft_and_barrier_energy(gpu_device, activation):
     assert abs(got[2] - diff) <= 1e-12 * (jd + curl) and abs(got[2] - (jd + curl)) <= 1e-12 * (jd + curl)
 
 
+@pytest.mark.parametrize("activation", [.05, .7])
+def test_trial_energy_with_a_quiet_base(gpu_device, activation):
+    """A base with no active closed27 point, its least J passed in: the sweep skips the base's closed27 J and gives
+    the same bits (activation .05: no active point anywhere; .7: the base, a copy of the reference, has none while
+    the trial has some). A passed minimum below the activation gives a NaN barrier value."""
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case quiet base", activation, flush=True)
+    prob, og, m = case(activation)
+    ref, base, u = fields(og.shape, 20)
+    q = FusedQ1(prob, gpu_device)
+    U, R = dev(u, gpu_device), dev(ref, gpu_device)
+    B = R.clone() if activation == .7 else dev(base, gpu_device)
+    full = q.trial_energy(U, R, B)
+    bmin = q.trial_energy(B, R)[4:5].clone()
+    assert float(bmin) >= q.activation and float(bmin) == float(full[6])
+    if activation == .7:
+        assert float(full[3]) > 0 and float(full[4]) < q.activation
+    for _ in range(2):
+        assert torch.equal(q.trial_energy(U, R, B, base_min=bmin), full)
+    low = torch.full((1,), q.activation / 2, dtype=torch.float64, device=gpu_device)
+    assert torch.isnan(q.trial_energy(U, R, B, base_min=low)[3])
+
+
 def test_energy_nan_propagates_to_min_j(gpu_device):
     from seq_vp_3d_amd.q1 import FusedQ1
     prob, og, m = case()

@@ -143,10 +143,12 @@ def load():
 
 
 class CallTimer:
-    """Records a pair of HIP events (on the current stream) around every C-ABI call while active."""
+    """Records a pair of HIP events (on the current stream) around every C-ABI call while active, or only around
+    the calls of the entry points in `names` (each bracket costs the host a few microseconds)."""
 
-    def __init__(self):
+    def __init__(self, names=None):
         self.events = {}
+        self.names = None if names is None else frozenset(names)
 
     def summary(self):
         """name -> (calls, mean ms) over the recorded calls (synchronizes on the last event)."""
@@ -164,6 +166,8 @@ TIMER = None
 
 def call(name, *args):
     timer = TIMER
+    if timer is not None and timer.names is not None and name not in timer.names:
+        timer = None
     if timer is not None and torch.cuda.is_current_stream_capturing():
         timer = None                # a captured call runs later, inside its graph: nothing to bracket now
     if timer is not None:

@@ -539,6 +539,11 @@ class PairState:
         separate slots of one small buffer. `ratio` reads the slot step_ratio_launch filled. With `fp32` not None
         the direction p = -S M^-1 (S g) (FP32 transforms when true) is computed first, and with `gradient` the
         gradient before it (gradient()); all of it runs as one segment."""
+        return self.direction_scalars_read(self.direction_scalars_launch(prev, norm, ratio, fp32, gradient))
+
+    def direction_scalars_launch(self, prev, norm=False, ratio=False, fp32=None, gradient=False):
+        """Enqueues direction_scalars' segment and returns at once; direction_scalars_read(pending) synchronises
+        and returns its values. The host may do other work in between, as long as it changes no state."""
         sweep = gradient and not self._take_outer_gradient()
         prev_key = None if prev is None else (prev["theta"], prev["g"], prev["p"])
         # the pending step ratio comes with the one-pass scalars when its pair is (theta, the history's theta), the
@@ -551,9 +556,13 @@ class PairState:
                      self.u, self.theta, self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale,
                      self._den, self.q1.ws.buf, self.dot_ws, self.scalars)
         self._segment(key, lambda: self._direction_segment(prev, fp32, sweep, extra))
+        return prev is not None, norm, ratio
+
+    def direction_scalars_read(self, pending):
+        has_prev, norm, ratio = pending
         self._wait()
         v = self._scalars_np.tolist()
-        bb2 = None if prev is None else (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7])
+        bb2 = (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7]) if has_prev else None
         G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None
         return v[0], bb2, v[8], G, (self._ratio(v[9], v[10]) if ratio else None)
 

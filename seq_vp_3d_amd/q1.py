@@ -74,8 +74,13 @@ class FusedQ1:
         self._check(u, ref) if base is None else self._check(u, ref, base)
         out = torch.empty(7, dtype=torch.float64, device=self.device) if out is None else out
         w, n = self._workspace(OP_TRIAL)
-        _lib.call("seqvp_q1_trial_energy_base_min", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(base_min),
-                  _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        if base_min is None:
+            _lib.call("seqvp_q1_trial_energy", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
+                      *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
+        else:
+            _lib.call("seqvp_q1_trial_energy_base_min", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base),
+                      _lib.ptr(base_min), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
+                      _lib.stream_handle(self.device))
         return out
 
     def soft_energy_f32(self, u, ref, base=None):

@@ -238,6 +238,7 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
         return InnerResult(status, reason, accepted_steps, G, trials, history)
 
     ratio_pending = False       # the last accepted step's ratio is on the device, read with the next scalars
+    prefetched = None           # the next iteration's direction scalars, enqueued at the acceptance
     while True:
         scalars = None
         try:
@@ -248,11 +249,15 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 st.outer_cache = st.outer_norms()
                 G = st.outer_cache[2]
             elif metric_ready:
-                # the gradient, the direction and its scalars are enqueued before the gradient checks and read with
-                # the gradient norm (and the last step's ratio) in one synchronisation; when a check below ends the
-                # solve they are simply not used
-                scalars = st.direction_scalars(previous, norm=True, ratio=ratio_pending,
-                                               fp32=config.fp32_direction, gradient=True)
+                # the gradient, the direction and its scalars are enqueued before the gradient checks (at the
+                # acceptance that ended the last search, when there was one) and read with the gradient norm (and
+                # the last step's ratio) in one synchronisation; when a check below ends the solve they are simply
+                # not used
+                if prefetched is not None:
+                    scalars, prefetched = st.direction_scalars_read(prefetched), None
+                else:
+                    scalars = st.direction_scalars(previous, norm=True, ratio=ratio_pending,
+                                                   fp32=config.fp32_direction, gradient=True)
                 G = scalars[3]
             else:
                 st.gradient()
@@ -365,11 +370,23 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 decrease = next_tau * abs(slope)
                 exit_info = dict(next_tau=next_tau, predicted_decrease=decrease, resolution_floor=exit_floor,
                                  stop=bool(decrease <= exit_floor))
+            energy_base = st.energy.total
+            if accepted:
+                # the step ratio (small-step count) is read at the next synchronisation, before any check uses it;
+                # the next iteration's gradient, direction and scalars are enqueued before the host records this
+                # attempt, so that the device works while it does
+                st.step_ratio_launch(st.theta_t, st.theta)
+                ratio_pending = True
+                previous = st.promote(tr.energy, keep_history=True)
+                previous["tau"] = tau
+                if metric_ready and accepted_steps + 1 < config.max_iterations:
+                    prefetched = st.direction_scalars_launch(previous, norm=True, ratio=True,
+                                                             fp32=config.fp32_direction, gradient=True)
             row = dict(iteration=accepted_steps, attempt=attempt + 1, accepted=bool(accepted), reason=reason, tau=tau,
                        step_proposal=proposal, step_rule=rule, untruncated=untruncated, domain_step_bound=bound,
                        gradient_norm=G, gradient_dot_direction=slope, trial_valid=bool(tr.ok and not tr.unproven_cells),
                        actual_reduction=actual,
-                       energy_resolution=resolution, certified_reduction=certified, energy_base=st.energy.total,
+                       energy_resolution=resolution, certified_reduction=certified, energy_base=energy_base,
                        energy_trial=tr.energy.total if tr.energy is not None else None,
                        unproven_cells=tr.unproven_cells, certificate=certificate_record(tr.certificate),
                        jd_prediction=prediction, resolution_exit=exit_info)
@@ -379,11 +396,6 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             if reason.startswith("NONFINITE"):
                 return finish("nonfinite_evaluation", reason)
             if accepted:
-                # the step ratio (small-step count) is read at the next synchronisation, before any check uses it
-                st.step_ratio_launch(st.theta_t, st.theta)
-                ratio_pending = True
-                previous = st.promote(tr.energy, keep_history=True)
-                previous["tau"] = tau
                 accepted_steps += 1
                 break
             global_attempts += 1

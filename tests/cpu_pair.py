@@ -121,6 +121,12 @@ class CpuPairState:
         return (slope, (None if prev is None else self.bb2_products(prev)), self.step_bound(),
                 self.gradient_norm() if norm else None, self._ratio if ratio else None)
 
+    def direction_scalars_launch(self, prev, norm=False, ratio=False, fp32=None, gradient=False):
+        return self.direction_scalars(prev, norm, ratio, fp32, gradient)
+
+    def direction_scalars_read(self, pending):
+        return pending
+
     def step_ratio_launch(self, a, b):
         self._ratio = self.step_ratio(a, b)
 

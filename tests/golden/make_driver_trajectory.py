@@ -1,13 +1,16 @@
-"""Writes tests/golden/driver_trajectory_32.json: the oracle driver's trajectory on the 32^3 CPU-sized pair.
+"""Writes tests/golden/driver_trajectory_32.json (10 outer steps) or driver_trajectory_32_<N>.json (N outer steps):
+the oracle driver's trajectory on the 32^3 CPU-sized pair.
 
 The pair is the 32x32x32 recipe (12 Gaussian blobs, sigma 2-6 voxels, 3 sinusoid modes per component, true JD >= 0.6,
 noise 0.01, seed 0) built with the oracle's spline sampler; the solve is oracle/driver.py's restatement of the
 reference's outer proximal loop and inner PG solve (sequential.py:441-689, poisson_gradient.py:1090-1646) with the
-reference's defaults, certified outer steps from u = 0 (10 stored). The oracle's NumPy sweeps take one to several
-minutes per outer step at this size, so the GPU test (tests/test_gpu_solver.py) reads the stored trajectory instead
-of recomputing it. Floats are stored as float.hex strings so the file carries every bit. Usage:
+reference's defaults, certified outer steps from u = 0. The oracle's NumPy sweeps take about a minute per outer
+step at this size (50 steps: 51 minutes on one core), so the GPU tests (tests/test_gpu_solver.py) read the stored
+trajectories instead of recomputing them. Floats are stored as float.hex strings so the file carries every bit.
+Usage:
 
-    python tests/golden/make_driver_trajectory.py [--outer 10]
+    python tests/golden/make_driver_trajectory.py [--outer 10]     # driver_trajectory_32.json
+    python tests/golden/make_driver_trajectory.py --outer 50       # driver_trajectory_32_50.json
 """
 import argparse
 import json
@@ -72,7 +75,7 @@ def main():
         theta=dict(index=index.tolist(), values=[enc(float(theta[i])) for i in index],
                    norm=enc(float(np.linalg.norm(theta)))),
     )
-    path = OUT / "driver_trajectory_32.json"
+    path = OUT / ("driver_trajectory_32.json" if args.outer == 10 else f"driver_trajectory_32_{args.outer}.json")
     path.write_text(json.dumps(doc, separators=(",", ":")) + "\n")
     print(f"wrote {path} ({path.stat().st_size} B): {out.status}, {len(out.history)} outer rows, "
           f"{len(out.inner_history)} attempts, {time.perf_counter() - t0:.1f} s")

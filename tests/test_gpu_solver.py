@@ -160,11 +160,7 @@ def _decode(v):
     return float.fromhex(v["hex"]) if isinstance(v, dict) else v
 
 
-def test_device_solver_follows_the_stored_oracle_trajectory(gpu_device):
-    """The 32^3 CPU-sized pair (12 blobs, true JD >= 0.6), the stored number of certified outer steps from u = 0
-    with the reference's defaults: the device driver on PairState against the oracle driver's trajectory stored by
-    tests/golden/make_driver_trajectory.py (every attempt's decision, tau and trial energy and every outer row to
-    1e-12 relative; 97 entries of the final theta)."""
+def _follow_stored_trajectory(gpu_device, name, late=None, theta_rel=1e-10):
     import json
     from pathlib import Path
     from types import SimpleNamespace
@@ -175,7 +171,7 @@ def test_device_solver_follows_the_stored_oracle_trajectory(gpu_device):
     from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
     from seq_vp_3d_amd.synthetic import CONFIG_SMALL, make_pair
 
-    path = Path(__file__).resolve().parent / "golden" / "driver_trajectory_32.json"
+    path = Path(__file__).resolve().parent / "golden" / name
     doc = json.loads(path.read_text())
     print("case solver vs stored oracle trajectory 32^3,", doc["config"]["outer"], "outer steps", flush=True)
 
@@ -195,13 +191,32 @@ def test_device_solver_follows_the_stored_oracle_trajectory(gpu_device):
     rows, inner_rows = [], []
     res = solve_sequential(gpu, SequentialConfig(), PGConfig(), max_outer=doc["config"]["outer"],
                            observer=rows.append, inner_observer=inner_rows.append)
-    compare_runs(want, want_inner, res, rows, inner_rows)
+    compare_runs(want, want_inner, res, rows, inner_rows, late=late)
     theta = gpu.theta.cpu().numpy()
     got = theta[np.asarray(doc["theta"]["index"])]
     ref = np.array([_decode(v) for v in doc["theta"]["values"]])
     scale = np.abs(ref).max()
-    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * scale)
-    assert abs(np.linalg.norm(theta) - _decode(doc["theta"]["norm"])) <= 1e-10 * _decode(doc["theta"]["norm"])
+    print("final theta: max deviation / max |theta| =", float(np.abs(got - ref).max() / scale), flush=True)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=theta_rel * scale)
+    assert abs(np.linalg.norm(theta) - _decode(doc["theta"]["norm"])) <= theta_rel * _decode(doc["theta"]["norm"])
+
+
+def test_device_solver_follows_the_stored_oracle_trajectory(gpu_device):
+    """The 32^3 CPU-sized pair (12 blobs, true JD >= 0.6), 10 certified outer steps from u = 0 with the reference's
+    defaults: the device driver on PairState against the oracle driver's trajectory stored by
+    tests/golden/make_driver_trajectory.py (every attempt's decision, tau and trial energy and every outer row to
+    1e-12 relative; 97 entries of the final theta to 1e-10 of its largest)."""
+    _follow_stored_trajectory(gpu_device, "driver_trajectory_32.json")
+
+
+def test_device_solver_follows_50_stored_oracle_outer_steps(gpu_device):
+    """The same pair for 50 outer steps (250 attempts): every decision (reason, acceptance, step rule, inner
+    status and step count) identical throughout. The values stay at the rounding floor (at most 1.2e-12 relative)
+    through outer step 37; from step 38 the solver amplifies the last-bit differences of the two implementations
+    (FMA contraction, summation order) by about 1.6x per step, to 2.1e-9 at step 49 (the BB2 step proposal, a
+    ratio of differences, is the most sensitive). Bars: 1e-12 for steps 0-9, 1e-11 for 10-37, 1e-8 from 38 on; the
+    final theta to 1e-10 of its largest entry (measured 3.3e-12)."""
+    _follow_stored_trajectory(gpu_device, "driver_trajectory_32_50.json", late=[(10, 1e-11), (38, 1e-8)])
 
 
 def test_first_inner_gradient_after_refresh_is_the_outer_gradient(gpu_device):

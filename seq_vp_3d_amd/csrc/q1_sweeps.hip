@@ -1321,8 +1321,58 @@ __device__ __forceinline__ void root_cubic(const double F[3][3], const double V[
   b3 = c3 * L3;
 }
 
+// Whether a cell's 27 cubics can be skipped, from its edges alone. Every point gradient is a blend of the four
+// parallel edges with weights >= 0 summing to 1, so for any centre c_rj, |grad u(p)_rj - c_rj| <= max_k |e_rjk - c_rj|
+// =: s_rj; with c the edges' mean, F(p) = Fc + E(p), Fc = I + c, |E(p)|_F <= S = |s|_F. Expanding det(Fc + E) like
+// the cubic itself, J(p) >= det Fc - |cof Fc| S - |Fc| S^2/sqrt3 - S^3/(3 sqrt3) =: Jlb (Frobenius norms;
+// |cof M|^2 = e2(sigma^2) <= |M|^4/3 and sigma1 sigma2 sigma3 <= (|M|^2/3)^(3/2)). With NF = |Fc| + S >= |F(p)| and
+// NV = |w| >= |V(p)| (w_rj = max_k |e'_rjk|, the direction's edges), the scaled coefficients obey |b1| <= L NF^2/sqrt3
+// NV, |b2| <= L^2 NF NV^2/sqrt3, |b3| <= L^3 NV^3/(3 sqrt3). When Jlb - delta exceeds their sum, no point has
+// c0 <= 0 and none a root in (0, 1]: the cubics would leave the block's results as they are. The margins (1e-9
+// relative on the sum, 1e-12 (1 + NF^3) absolute) cover the rounding of these bounds and of the cubics' own
+// evaluation; a nonfinite edge (the 0 * e probe) or an overflow keeps the cell.
+__device__ __forceinline__ bool root_free_cell(const Edges& Eu, const Edges& Ev, double delta, double L) {
+  double F[3][3], S2 = 0.0, nv2 = 0.0, probe = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double* e = Eu.e[j][r];
+      const double m = 0.25 * ((e[0] + e[1]) + (e[2] + e[3]));
+      double sp = 0.0, w = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sp = fmax(sp, fabs(e[k] - m));
+        w = fmax(w, fabs(Ev.e[j][r][k]));
+        probe = fma(e[k], 0.0, probe);
+        probe = fma(Ev.e[j][r][k], 0.0, probe);
+      }
+      F[r][j] = (r == j ? 1.0 : 0.0) + m;
+      S2 = fma(sp, sp, S2);
+      nv2 = fma(w, w, nv2);
+    }
+  double A[3][3];
+  cofactor_plain(F, A);
+  const double det = (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
+  double nf2 = 0.0, na2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nf2 = fma(F[r][j], F[r][j], nf2);
+      na2 = fma(A[r][j], A[r][j], na2);
+    }
+  const double inv_sqrt3 = 0.57735026918962584, inv_3sqrt3 = 0.19245008972987526;
+  const double S = sqrt(S2), nfc = sqrt(nf2), nv = sqrt(nv2);
+  const double jlb = det - (sqrt(na2) * S + nfc * S2 * inv_sqrt3 + S2 * S * inv_3sqrt3);
+  const double NF = nfc + S;
+  const double bound = L * (NF * NF * inv_sqrt3 * nv + L * (NF * nv2 * inv_sqrt3 + L * nv2 * nv * inv_3sqrt3));
+  return probe == 0.0 && (jlb - delta) > bound * (1.0 + 1e-9) + 1e-12 * (1.0 + NF * NF * NF);
+}
+
 // Per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129; torch_ops.py:636-673).
 // FREE: v is the direction's free-node vector (the node values of inject(v), read without the node field).
+// A wave whose cells all pass root_free_cell evaluates no cubic (the same results).
 template <bool FREE>
 __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict__ u, const double* __restrict__ v,
                                                         const double* __restrict__ k, double L, Dims d,
@@ -1348,10 +1398,11 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
         load_cell(v, b, d, w);
       make_edges(w, ih, Ev);
     }
+    const bool skip = root_free_cell(Eu, Ev, delta, L);
     // Pass 1: every point with its compile-time closed blend; nonfinite and c0 <= 0 points are counted, and a
     // point whose cubic may have a root in (0, 1] marks the lane.
     bool need = false;
-    static_for<27>([&](auto P) {
+    if (!__all(skip)) static_for<27>([&](auto P) {
       if constexpr (P % 9 == 0) {
         opaque_edges(Eu);
         opaque_edges(Ev);

@@ -145,9 +145,9 @@ class PairState:
         self._tau_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
         self._tau_dev = z(1)
         self._trial_dev = z(17)
-        # least closed27 J of the accepted state when known (host float and device copy): at or above the
-        # activation, the trial sweep needs no closed27 J of the accepted state
-        self._base_min, self._base_min_dev, self._trial_min = None, z(1), None
+        # least closed27 J of the accepted state when known (host float and device copy, the copy written before
+        # its next use): at or above the activation, the trial sweep needs no closed27 J of the accepted state
+        self._base_min, self._base_min_dev, self._base_min_stale, self._trial_min = None, z(1), False, None
         self._trial_host = torch.zeros(17, dtype=torch.float64, pin_memory=True)
         self._scalars_host = torch.zeros(12, dtype=torch.float64, pin_memory=True)
         self._tau_np, self._trial_np, self._scalars_np = (t.numpy() for t in (self._tau_host, self._trial_host,
@@ -214,8 +214,17 @@ class PairState:
 
     def _set_base_min(self, value):
         self._base_min = value
-        if self._quiet():
-            self._base_min_dev.fill_(value)
+        self._base_min_stale = value is not None
+
+    def _base_min_ptr(self):
+        """The device double holding the accepted state's least closed27 J, written now if it changed (None when
+        unknown). Called outside captured segments: a graph replays the value it captured."""
+        if self._base_min is None:
+            return None
+        if self._base_min_stale:
+            self._base_min_dev.fill_(self._base_min)
+            self._base_min_stale = False
+        return self._base_min_dev
 
     def _quiet(self):
         """Whether the accepted state has no active closed27 point (known from its last evaluation)."""
@@ -362,10 +371,11 @@ class PairState:
         # needs no second warp at promotion), the energies, and one copy of the 17 scalars to the host
         self._tau_np[0] = tau
         batched = self.batch is not None
-        key = key_of("trial", batched, self._quiet(), self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
+        quiet = self._quiet() and self._base_min_ptr() is not None
+        key = key_of("trial", batched, quiet, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
                      self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
                      self.cert.capacity)
-        self._segment(key, lambda: self._trial_segment(not batched))
+        self._segment(key, lambda: self._trial_segment(not batched, quiet))
         if batched:
             # the soft and barrier sweep in the group's multi-pair launch, then the scalars' copy
             self.batch.trial_energy(self, self._trial_dev[4:11])
@@ -378,7 +388,7 @@ class PairState:
         self._trial_min = vals[8]
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
 
-    def _trial_segment(self, energies=True):
+    def _trial_segment(self, energies=True, quiet=False):
         nz, ny, nx = self.shape
         q, w, s = self.q1, self.problem.weights, self._stream()
         d = self._trial_dev
@@ -391,7 +401,7 @@ class PairState:
                       base_warped=self.warped, base_u=self.u, out=d[0:4])
         if energies:
             q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11],      # soft and barrier terms in one sweep
-                           base_min=self._base_min_dev if self._quiet() else None)
+                           base_min=self._base_min_dev if quiet else None)
             _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
 
     def trial_energies(self, screen=False):
@@ -404,7 +414,7 @@ class PairState:
             sb = torch.cat((q.soft_energy_f32(self.u_t, self.ref, self.u), q.barrier_energy_f32(self.u_t, self.u)))
         else:
             sb = q.trial_energy(self.u_t, self.ref, self.u,       # soft and barrier terms in one sweep
-                                base_min=self._base_min_dev if self._quiet() else None)
+                                base_min=self._base_min_ptr() if self._quiet() else None)
         vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
         self._trial_min = None if screen else vals[8]      # the FP32 screen's min J is not the closed27 J
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)

@@ -180,6 +180,47 @@ def test_jd_root(gpu_device):
             root(dev(base, gpu_device), dev(vn, gpu_device), limit), limit
 
 
+@pytest.mark.parametrize("amp", [.02, .3, 1.5])
+def test_jd_root_where_most_cells_are_skipped(gpu_device, amp):
+    """The root sweep skips the cells its edge bound shows root-free: directions small everywhere but for a local
+    bump (most waves skipped, the bump's cells evaluated) give the oracle's root (1e-12) and counts exactly; the free
+    vector gives the node field's results bit for bit; a NaN in the direction sets the nonfinite flag."""
+    from seq_vp_3d_amd.q1 import FusedJDRoot, FusedQ1
+    print("case jd root with skipped cells", amp, flush=True)
+    prob, og, m = case(shape=(17, 18, 19))
+    _, base, _ = fields(og.shape, 70)
+    v = smooth_field(og.shape, .01, 73)
+    nz, ny, nx = og.shape
+    z, y, x = np.meshgrid(*(np.arange(n) for n in og.shape), indexing="ij")
+    bump = np.exp(-((z - nz / 3) ** 2 + (y - ny / 2) ** 2 + (x - 2 * nx / 3) ** 2) / 4.)
+    v += amp * bump[..., None] * np.array([1., -.6, .4])
+    v[[0, -1]] = 0.
+    v[:, [0, -1]] = 0.
+    v[:, :, [0, -1]] = 0.
+    q = FusedQ1(prob, gpu_device)
+    root = FusedJDRoot(prob, q1=q)
+    B = dev(base, gpu_device)
+    pf = Q.restrict_free(v, og)
+    vn = Q.inject_free(pf, og)
+    found = 0
+    for limit in (.05, .5, 2., 10.):
+        got = root(B, dev(vn, gpu_device), limit)
+        want = ojr.global_root(base, vn, og, m.closed[3], 0., limit)
+        assert got[1:] == want[1:]
+        if np.isinf(want[0]):
+            assert np.isinf(got[0])
+        else:
+            found += 1
+            assert abs(got[0] - want[0]) <= 1e-12 * want[0], (limit, got, want)
+        assert root.from_free(B, dev(pf, gpu_device), limit) == got, limit
+    if amp == 1.5:
+        assert found >= 2        # roots inside the bump at the larger limits
+    pn = pf.copy()
+    pn[5] = np.nan
+    out = root.from_free_launch(B, dev(pn, gpu_device), 1.)
+    assert float(out[2]) == 1.
+
+
 def test_prescreen_bitwise(gpu_device):
     from seq_vp_3d_amd.prescreen import prescreen_flags, prescreen_lower_bounds
     import oracle.build

@@ -806,10 +806,72 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, 
   }
 }
 
-__device__ __forceinline__ void barrier_vjp(Edges& Eu, const double* __restrict__ k, EdgeAdj& G) {
+// cofactor rows of a plain 3x3 (no identity): R_r = M_{r+1} x M_{r+2} (geometry.py:11-23).
+__device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int j = (r + 1) % 3, kk = (r + 2) % 3;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int l = (cc + 1) % 3, m = (cc + 2) % 3;
+      A[r][cc] = M[j][l] * M[kk][m] - M[j][m] * M[kk][l];
+    }
+  }
+}
+
+// A lower bound of J = det(I + grad u) over a cell's closed points, from its edges alone. Every point gradient is a
+// blend of the four parallel edges with weights >= 0 summing to 1, so for any centre c_rj, |grad u(p)_rj - c_rj| <=
+// max_k |e_rjk - c_rj| =: s_rj; with c the edges' mean, F(p) = Fc + E(p), Fc = I + c, |E(p)|_F <= S = |s|_F.
+// Expanding det(Fc + E) like a cubic in E, J(p) >= det Fc - |cof Fc| S - |Fc| S^2/sqrt3 - S^3/(3 sqrt3) (Frobenius
+// norms; |cof M|^2 = e2(sigma^2) <= |M|^4/3 and sigma1 sigma2 sigma3 <= (|M|^2/3)^(3/2)). nf: |Fc| + S >= |F(p)|.
+// A nonfinite edge makes the bound or nf NaN or infinite, so every comparison a caller makes with it fails.
+__device__ __forceinline__ double cell_j_lower_bound(const Edges& E, double& nf) {
+  double F[3][3], S2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double* e = E.e[j][r];
+      const double m = 0.25 * ((e[0] + e[1]) + (e[2] + e[3]));
+      double sp = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sp = fmax(sp, fabs(e[k] - m));
+      F[r][j] = (r == j ? 1.0 : 0.0) + m;
+      S2 = fma(sp, sp, S2);
+    }
+  double A[3][3];
+  cofactor_plain(F, A);
+  const double det = (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
+  double nf2 = 0.0, na2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nf2 = fma(F[r][j], F[r][j], nf2);
+      na2 = fma(A[r][j], A[r][j], na2);
+    }
+  const double inv_sqrt3 = 0.57735026918962584, inv_3sqrt3 = 0.19245008972987526;
+  const double S = sqrt(S2), nfc = sqrt(nf2);
+  nf = nfc + S;
+  return det - (sqrt(na2) * S + nfc * S2 * inv_sqrt3 + S2 * S * inv_3sqrt3);
+}
+
+// Whether every closed point of the cell has J >= level, with a margin (1e-12 (1 + nf^3)) that covers the rounding
+// of the bound and of the sweeps' own J: a cell that passes has no point below the barrier activation, so its
+// barrier terms (value, change, gradient, curvature) are exactly zero.
+__device__ __forceinline__ bool cell_j_above(const Edges& E, double level) {
+  double nf;
+  const double jlb = cell_j_lower_bound(E, nf);
+  return (jlb - level) > 1e-12 * (1.0 + nf * nf * nf);
+}
+
+// quiet: the cell's edges bound J at or above the activation at every closed point (cell_j_above).
+__device__ __forceinline__ void barrier_vjp(Edges& Eu, const double* __restrict__ k, EdgeAdj& G, bool quiet) {
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
-  // psi'(J) is exactly 0 at and above the activation: J at the 27 points with their compile-time closed blends
-  // first, and a wave with no point below it contributes nothing.
+  // psi'(J) is exactly 0 at and above the activation: a wave whose cells are all quiet contributes nothing; else J
+  // at the 27 points with their compile-time closed blends first, and a wave with no point below it contributes
+  // nothing either.
+  if (__all(quiet)) return;
   bool active = false;
   static_for<27>([&](auto P) {
     if constexpr (P % 9 == 0) opaque_edges(Eu);
@@ -868,6 +930,9 @@ __global__ void __launch_bounds__(BLOCK, 2) grad_locals_kernel(const double* __r
     load_cell(u, b, d, v);
     make_edges(v, ih, Eu);
   }
+  // whether the barrier terms vanish on the whole cell, from its edges (before the soft pass, while few registers
+  // are live)
+  const bool quiet = BARRIER && cell_j_above(Eu, k[K_ACT]);
   EdgeAdj G;
   adj_zero(G);
   if (SOFT) {
@@ -883,7 +948,7 @@ __global__ void __launch_bounds__(BLOCK, 2) grad_locals_kernel(const double* __r
     adj_store(G, ih, loc, c, xc, valid, d);
     adj_zero(G);
   }
-  if (BARRIER) barrier_vjp(Eu, k, G);
+  if (BARRIER) barrier_vjp(Eu, k, G, quiet);
   adj_store(G, ih, MODE == 2 ? loc2 : loc, c, xc, valid, d);
 }
 
@@ -918,6 +983,7 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
     load_cell(u, b, d, v);
     make_edges(v, ih, Eu);
   }
+  const bool quiet = cell_j_above(Eu, act);
   double acc[8][3];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
@@ -944,10 +1010,11 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
       }
     }
   }
-  // psi''(J) is exactly 0 at and above the activation: J at the 27 points with their compile-time closed blends
-  // first, and a wave with no point below it skips the barrier part.
+  // psi''(J) is exactly 0 at and above the activation: a wave whose cells' edges bound J at or above it at every
+  // closed point (cell_j_above) skips the barrier part; else J at the 27 points with their compile-time closed
+  // blends first, and a wave with no point below it skips it too.
   bool active = false;
-  static_for<27>([&](auto P) {
+  if (!__all(quiet)) static_for<27>([&](auto P) {
     if constexpr (P % 9 == 0) opaque_edges(Eu);
     double V[3][3];
     closed_point_grad<P>(Eu, V);
@@ -1286,18 +1353,6 @@ __device__ double first_root(double b0, double b1, double b2, double b3) {
   return lo;
 }
 
-// cofactor rows of a plain 3x3 (no identity): R_r = M_{r+1} x M_{r+2} (geometry.py:11-23).
-__device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3][3]) {
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int j = (r + 1) % 3, kk = (r + 2) % 3;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      const int l = (cc + 1) % 3, m = (cc + 2) % 3;
-      A[r][cc] = M[j][l] * M[kk][m] - M[j][m] * M[kk][l];
-    }
-  }
-}
 
 // The T66 cubic det(F + s V) - delta = c0 + c1 s + c2 s^2 + c3 s^3 at one point (F = I + grad u, V = grad v),
 // scaled to s/L: b_i = c_i L^i (global_jd_step.py:90-129).
@@ -1321,51 +1376,30 @@ __device__ __forceinline__ void root_cubic(const double F[3][3], const double V[
   b3 = c3 * L3;
 }
 
-// Whether a cell's 27 cubics can be skipped, from its edges alone. Every point gradient is a blend of the four
-// parallel edges with weights >= 0 summing to 1, so for any centre c_rj, |grad u(p)_rj - c_rj| <= max_k |e_rjk - c_rj|
-// =: s_rj; with c the edges' mean, F(p) = Fc + E(p), Fc = I + c, |E(p)|_F <= S = |s|_F. Expanding det(Fc + E) like
-// the cubic itself, J(p) >= det Fc - |cof Fc| S - |Fc| S^2/sqrt3 - S^3/(3 sqrt3) =: Jlb (Frobenius norms;
-// |cof M|^2 = e2(sigma^2) <= |M|^4/3 and sigma1 sigma2 sigma3 <= (|M|^2/3)^(3/2)). With NF = |Fc| + S >= |F(p)| and
-// NV = |w| >= |V(p)| (w_rj = max_k |e'_rjk|, the direction's edges), the scaled coefficients obey |b1| <= L NF^2/sqrt3
-// NV, |b2| <= L^2 NF NV^2/sqrt3, |b3| <= L^3 NV^3/(3 sqrt3). When Jlb - delta exceeds their sum, no point has
-// c0 <= 0 and none a root in (0, 1]: the cubics would leave the block's results as they are. The margins (1e-9
-// relative on the sum, 1e-12 (1 + NF^3) absolute) cover the rounding of these bounds and of the cubics' own
-// evaluation; a nonfinite edge (the 0 * e probe) or an overflow keeps the cell.
+// Whether a cell's 27 T66 cubics can be skipped, from its edges alone: with Jlb and NF from cell_j_lower_bound for
+// u, and NV = |w| >= |V(p)| (w_rj = max_k |e'_rjk|, the direction's edges), the scaled coefficients obey |b1| <=
+// L NF^2/sqrt3 NV, |b2| <= L^2 NF NV^2/sqrt3, |b3| <= L^3 NV^3/(3 sqrt3). When Jlb - delta exceeds their sum, no
+// point has c0 <= 0 and none a root in (0, 1]: the cubics would leave the block's results as they are. The margins
+// (1e-9 relative on the sum, 1e-12 (1 + NF^3) absolute) cover the rounding of these bounds and of the cubics' own
+// evaluation; a nonfinite edge (u's through the bound, v's through the 0 * e probe) keeps the cell.
 __device__ __forceinline__ bool root_free_cell(const Edges& Eu, const Edges& Ev, double delta, double L) {
-  double F[3][3], S2 = 0.0, nv2 = 0.0, probe = 0.0;
+  double NF;
+  const double jlb = cell_j_lower_bound(Eu, NF);
+  double nv2 = 0.0, probe = 0.0;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const double* e = Eu.e[j][r];
-      const double m = 0.25 * ((e[0] + e[1]) + (e[2] + e[3]));
-      double sp = 0.0, w = 0.0;
+      double w = 0.0;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        sp = fmax(sp, fabs(e[k] - m));
         w = fmax(w, fabs(Ev.e[j][r][k]));
-        probe = fma(e[k], 0.0, probe);
         probe = fma(Ev.e[j][r][k], 0.0, probe);
       }
-      F[r][j] = (r == j ? 1.0 : 0.0) + m;
-      S2 = fma(sp, sp, S2);
       nv2 = fma(w, w, nv2);
     }
-  double A[3][3];
-  cofactor_plain(F, A);
-  const double det = (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
-  double nf2 = 0.0, na2 = 0.0;
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      nf2 = fma(F[r][j], F[r][j], nf2);
-      na2 = fma(A[r][j], A[r][j], na2);
-    }
   const double inv_sqrt3 = 0.57735026918962584, inv_3sqrt3 = 0.19245008972987526;
-  const double S = sqrt(S2), nfc = sqrt(nf2), nv = sqrt(nv2);
-  const double jlb = det - (sqrt(na2) * S + nfc * S2 * inv_sqrt3 + S2 * S * inv_3sqrt3);
-  const double NF = nfc + S;
+  const double nv = sqrt(nv2);
   const double bound = L * (NF * NF * inv_sqrt3 * nv + L * (NF * nv2 * inv_sqrt3 + L * nv2 * nv * inv_3sqrt3));
   return probe == 0.0 && (jlb - delta) > bound * (1.0 + 1e-9) + 1e-12 * (1.0 + NF * NF * NF);
 }

@@ -27,10 +27,18 @@ enum : int {
 __device__ constexpr double GAUSS_S[3] = {0x1.cda042f0236e0p-4, 0.5, 0x1.c64bf7a1fb924p-1};
 __device__ constexpr double CLOSED_S[3] = {0.0, 0.5, 1.0};
 
+// Division of an index n < 2^32 by a divisor 2 <= v < 2^32 as one multiply-high: with M = floor((2^64 - 1)/v) + 1,
+// floor(n/v) = floor(M n / 2^64) for every such n (Lemire, Kaser and Kurz, "Faster remainder by direct computation",
+// 2019, theorem 1 with 32-bit operands and a 64-bit multiplier). 0 marks a divisor the form does not cover.
+__host__ __device__ inline uint64_t div_magic(int64_t v) {
+  return (v >= 2 && v < ((int64_t)1 << 32)) ? UINT64_MAX / (uint64_t)v + 1 : 0;
+}
+
 struct Dims {
   int64_t nx, ny, nz;   // nodes
   int64_t cx, cy, cz;   // cells
   int64_t ncell;
+  uint64_t mcx, mcy;    // div_magic(cx), div_magic(cy); both nonzero and ncell < 2^32: cells split by multiplies
 };
 
 __host__ __device__ inline Dims make_dims(int64_t nx, int64_t ny, int64_t nz) {
@@ -38,7 +46,27 @@ __host__ __device__ inline Dims make_dims(int64_t nx, int64_t ny, int64_t nz) {
   d.nx = nx; d.ny = ny; d.nz = nz;
   d.cx = nx - 1; d.cy = ny - 1; d.cz = nz - 1;
   d.ncell = d.cx * d.cy * d.cz;
+  const bool fast = d.ncell < ((int64_t)1 << 32);
+  d.mcx = fast ? div_magic(d.cx) : 0;
+  d.mcy = fast ? div_magic(d.cy) : 0;
+  if (!d.mcx || !d.mcy) d.mcx = d.mcy = 0;
   return d;
+}
+
+// Cell c -> (x0, y0, z0). The test on the magics is uniform over a launch (a kernel argument).
+__device__ __forceinline__ void cell_coords(int64_t c, const Dims& d, int64_t& x0, int64_t& y0, int64_t& z0) {
+  if (d.mcx != 0) {
+    const uint64_t n = (uint32_t)c;
+    const uint64_t q = __umul64hi(d.mcx, n);
+    const uint64_t q2 = __umul64hi(d.mcy, q);
+    x0 = (int64_t)(n - q * (uint64_t)d.cx);
+    y0 = (int64_t)(q - q2 * (uint64_t)d.cy);
+    z0 = (int64_t)q2;
+  } else {
+    x0 = c % d.cx;
+    y0 = (c / d.cx) % d.cy;
+    z0 = c / (d.cx * d.cy);
+  }
 }
 
 // Node index of vertex a of the cell whose vertex 0 is node b.
@@ -47,9 +75,8 @@ __device__ __forceinline__ int64_t vertex_node(int64_t b, int a, const Dims& d) 
 }
 
 __device__ __forceinline__ int64_t cell_base_node(int64_t c, const Dims& d) {
-  const int64_t x0 = c % d.cx;
-  const int64_t y0 = (c / d.cx) % d.cy;
-  const int64_t z0 = c / (d.cx * d.cy);
+  int64_t x0, y0, z0;
+  cell_coords(c, d, x0, y0, z0);
   return (z0 * d.ny + y0) * d.nx + x0;
 }
 
@@ -85,12 +112,17 @@ __device__ __forceinline__ void load_cell(const double* __restrict__ f, int64_t 
 __device__ __forceinline__ void load_cell_free(const double* __restrict__ p, int64_t c, const Dims& d,
                                                double v[8][3]) {
   const int64_t mx = d.nx - 2, my = d.ny - 2, F = mx * my * (d.nz - 2);
-  const int64_t x0 = c % d.cx, y0 = (c / d.cx) % d.cy, z0 = c / (d.cx * d.cy);
+  int64_t x0, y0, z0;
+  cell_coords(c, d, x0, y0, z0);
+  // vertex (x0 + ax, y0 + ay, z0 + az) is a free node when each coordinate is strictly inside; its free index is
+  // the lower vertex's plus ax + ay mx + az mx my
+  const bool in_x[2] = {x0 > 0, x0 < d.nx - 2}, in_y[2] = {y0 > 0, y0 < d.ny - 2}, in_z[2] = {z0 > 0, z0 < d.nz - 2};
+  const int64_t f0 = ((z0 - 1) * my + (y0 - 1)) * mx + (x0 - 1);
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
-    const int64_t x = x0 + (a & 1), y = y0 + ((a >> 1) & 1), z = z0 + ((a >> 2) & 1);
-    const bool inner = x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1;
-    const int64_t f = inner ? ((z - 1) * my + (y - 1)) * mx + (x - 1) : 0;
+    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+    const bool inner = in_x[ax] && in_y[ay] && in_z[az];
+    const int64_t f = inner ? f0 + ax + ay * mx + az * (mx * my) : 0;
 #pragma unroll
     for (int r = 0; r < 3; ++r) v[a][r] = inner ? p[r * F + f] : 0.0;
   }

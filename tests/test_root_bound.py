@@ -118,3 +118,31 @@ def test_skipped_cells_have_no_root(amp, limit, seed):
     print("skipped fraction", skip.mean())
     if amp == .05:
         assert skip.mean() > .5          # the bound is not vacuous for small steps
+
+
+def test_index_division_by_multiply_high():
+    """The cell index split of the sweeps (div_magic and cell_coords in q1_common.h): floor(n / v) as the high 64
+    bits of M n, M = floor((2^64 - 1) / v) + 1, for 32-bit n and 2 <= v < 2^32, checked exactly with Python integers
+    on the divisors of the native grids and others, at the multiples of v, their neighbours, the top of the range and
+    random indices."""
+    rng = np.random.default_rng(9)
+    divisors = [2, 3, 5, 7, 157, 158, 159, 189, 190, 191, 221, 222, 223, 224, 1 << 16, (1 << 16) + 1, 65535,
+                (1 << 31) - 1, 1 << 31, (1 << 32) - 1] + [int(x) for x in rng.integers(2, 1 << 32, size=40)]
+    top = (1 << 32) - 1
+    for v in divisors:
+        m = ((1 << 64) - 1) // v + 1
+        assert 0 < m < (1 << 64)
+        ns = {0, 1, v - 1, v, v + 1, top, top - 1, top // v * v, top // v * v - 1}
+        ns |= {k * v + o for k in (1, 2, 3, 1000, top // v - 1) for o in (-1, 0, 1) if 0 <= k * v + o <= top}
+        ns |= {int(x) for x in rng.integers(0, top, size=2000, endpoint=True)}
+        for n in ns:
+            assert (m * n) >> 64 == n // v, (v, n)
+    # the node grids' cell split: x, y, z of cell c for a 160x192x224 node grid, by multiplies, at both ends and
+    # random cells
+    cx, cy, cz = 223, 191, 159
+    mcx, mcy = ((1 << 64) - 1) // cx + 1, ((1 << 64) - 1) // cy + 1
+    for c in [0, cx - 1, cx, cx * cy - 1, cx * cy, cx * cy * cz - 1] + [int(x) for x in rng.integers(0, cx * cy * cz,
+                                                                                                        size=2000)]:
+        q = (mcx * c) >> 64
+        q2 = (mcy * q) >> 64
+        assert (c - q * cx, q - q2 * cy, q2) == (c % cx, (c // cx) % cy, c // (cx * cy))

@@ -1379,12 +1379,10 @@ __device__ __forceinline__ void root_cubic(const double F[3][3], const double V[
 // Whether a cell's 27 T66 cubics can be skipped, from its edges alone: with Jlb and NF from cell_j_lower_bound for
 // u, and NV = |w| >= |V(p)| (w_rj = max_k |e'_rjk|, the direction's edges), the scaled coefficients obey |b1| <=
 // L NF^2/sqrt3 NV, |b2| <= L^2 NF NV^2/sqrt3, |b3| <= L^3 NV^3/(3 sqrt3). When Jlb - delta exceeds their sum, no
-// point has c0 <= 0 and none a root in (0, 1]: the cubics would leave the block's results as they are. The margins
+// point has c0 <= 0 and none a root in (0, 1]: the cubics would leave the sweep's results as they are. The margins
 // (1e-9 relative on the sum, 1e-12 (1 + NF^3) absolute) cover the rounding of these bounds and of the cubics' own
 // evaluation; a nonfinite edge (u's through the bound, v's through the 0 * e probe) keeps the cell.
-__device__ __forceinline__ bool root_free_cell(const Edges& Eu, const Edges& Ev, double delta, double L) {
-  double NF;
-  const double jlb = cell_j_lower_bound(Eu, NF);
+__device__ __forceinline__ bool root_free_cell(double jlb, double NF, const Edges& Ev, double delta, double L) {
   double nv2 = 0.0, probe = 0.0;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -1404,18 +1402,67 @@ __device__ __forceinline__ bool root_free_cell(const Edges& Eu, const Edges& Ev,
   return probe == 0.0 && (jlb - delta) > bound * (1.0 + 1e-9) + 1e-12 * (1.0 + NF * NF * NF);
 }
 
-// Per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129; torch_ops.py:636-673).
+template <bool FREE>
+__device__ __forceinline__ void load_direction(const double* __restrict__ v, int64_t c, int64_t b, const Dims& d,
+                                               double w[8][3]) {
+  if (FREE)
+    load_cell_free(v, c, d, w);
+  else
+    load_cell(v, b, d, w);
+}
+
+// The T66 sweep in two launches. The screen (one lane per cell, few registers, three waves per SIMD) lists every
+// wave with a cell that root_free_cell cannot clear; the cubic pass then evaluates only the listed waves. A cleared
+// cell has no root, no c0 <= 0 point and no nonfinite coefficient, so the listed waves alone give the sweep's
+// results; min, integer counts (exact in double) and max do not depend on the list's order.
 // FREE: v is the direction's free-node vector (the node values of inject(v), read without the node field).
-// A wave whose cells all pass root_free_cell evaluates no cubic (the same results).
+template <bool FREE>
+__global__ void __launch_bounds__(BLOCK, 3) jd_root_screen_kernel(const double* __restrict__ u,
+                                                                  const double* __restrict__ v,
+                                                                  const double* __restrict__ k, double L, Dims d,
+                                                                  unsigned long long* __restrict__ count,
+                                                                  int64_t* __restrict__ list) {
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+  bool skip = true;
+  if (c < d.ncell) {
+    double ih[3];
+    load_ih(k, ih);
+    const int64_t b = cell_base_node(c, d);
+    double jlb, NF;
+    {
+      Edges Eu;
+      double w[8][3];
+      load_cell(u, b, d, w);
+      make_edges(w, ih, Eu);
+      jlb = cell_j_lower_bound(Eu, NF);
+    }
+    Edges Ev;
+    {
+      double w[8][3];
+      load_direction<FREE>(v, c, b, d, w);
+      make_edges(w, ih, Ev);
+    }
+    skip = root_free_cell(jlb, NF, Ev, k[K_DELTA], L);
+  }
+  if (!__all(skip) && (threadIdx.x & 63) == 0) list[atomicAdd(count, 1ull)] = c >> 6;
+}
+
+// The cubic pass over the listed waves (wave w: cells 64 w .. 64 w + 63), each wave of the grid taking list entries
+// with a stride; per-block [min root, count c0 <= 0, any nonfinite] (global_jd_step.py:90-129;
+// torch_ops.py:636-673).
 template <bool FREE>
 __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict__ u, const double* __restrict__ v,
                                                         const double* __restrict__ k, double L, Dims d,
+                                                        const unsigned long long* __restrict__ count,
+                                                        const int64_t* __restrict__ list,
                                                         double* __restrict__ partials) {
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t c = blk * BLOCK + threadIdx.x;
   double best = __builtin_inf();
   double invalid = 0.0, bad = 0.0;
-  if (c < d.ncell) {
+  const int64_t listed = (int64_t)*count;
+  const int64_t waves = (int64_t)gridDim.x * (BLOCK / 64);
+  for (int64_t i = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); i < listed; i += waves) {
+    const int64_t c = list[i] * 64 + (threadIdx.x & 63);
+    if (c >= d.ncell) continue;
     double ih[3];
     load_ih(k, ih);
     const double delta = k[K_DELTA];
@@ -1426,17 +1473,13 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
       double w[8][3];
       load_cell(u, b, d, w);
       make_edges(w, ih, Eu);
-      if (FREE)
-        load_cell_free(v, c, d, w);
-      else
-        load_cell(v, b, d, w);
+      load_direction<FREE>(v, c, b, d, w);
       make_edges(w, ih, Ev);
     }
-    const bool skip = root_free_cell(Eu, Ev, delta, L);
     // Pass 1: every point with its compile-time closed blend; nonfinite and c0 <= 0 points are counted, and a
     // point whose cubic may have a root in (0, 1] marks the lane.
     bool need = false;
-    if (!__all(skip)) static_for<27>([&](auto P) {
+    static_for<27>([&](auto P) {
       if constexpr (P % 9 == 0) {
         opaque_edges(Eu);
         opaque_edges(Ev);
@@ -1505,11 +1548,13 @@ __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict
       i0 += sh[w][1];
       f0 = fmax(f0, sh[w][2]);
     }
-    partials[blk * 3 + 0] = b0;
-    partials[blk * 3 + 1] = i0;
-    partials[blk * 3 + 2] = f0;
+    partials[blockIdx.x * 3 + 0] = b0;
+    partials[blockIdx.x * 3 + 1] = i0;
+    partials[blockIdx.x * 3 + 2] = f0;
   }
 }
+
+__global__ void root_count_zero_kernel(unsigned long long* __restrict__ count) { *count = 0ull; }
 
 __global__ void __launch_bounds__(RB) jd_root_reduce_kernel(const double* __restrict__ partials, int64_t nblocks,
                                                                double* __restrict__ out) {
@@ -1710,6 +1755,22 @@ using namespace seqvp;
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// The T66 sweep's workspace: per-block partials of the cubic pass, the listed-wave count, the list.
+struct RootWorkspace {
+  int64_t eval_blocks;       // the cubic pass's grid: at most two workgroups per CU, at most one wave per list entry
+  size_t count_offset, list_offset, bytes;
+};
+
+static RootWorkspace root_workspace(const Dims& d) {
+  RootWorkspace r;
+  const int64_t waves = cdiv(d.ncell, 64);
+  r.eval_blocks = std::max<int64_t>(1, std::min<int64_t>(cdiv(waves, BLOCK / 64), 512));
+  r.count_offset = (size_t)r.eval_blocks * 3 * sizeof(double);
+  r.list_offset = r.count_offset + 256;
+  r.bytes = r.list_offset + (size_t)waves * sizeof(int64_t);
+  return r;
+}
+
 static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
   const Dims d = make_dims(nx, ny, nz);
   const int64_t nblocks = cdiv(d.ncell, BLOCK);
@@ -1719,7 +1780,7 @@ static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
     case 2: return (size_t)d.ncell * 24 * sizeof(double);
     case 3: return (size_t)d.ncell * 24 * sizeof(double);
     case 4: return (size_t)d.ncell * 48 * sizeof(double);
-    case 5: return (size_t)nblocks * 3 * sizeof(double);
+    case 5: return root_workspace(d).bytes;
     case 8: return (size_t)d.ncell * sizeof(double);
     case 9: return (size_t)1024 * 8 * sizeof(double);
     case 10: return (size_t)nblocks * 12 * sizeof(double);
@@ -2064,30 +2125,34 @@ extern "C" int seqvp_face_interior_minima_host(const double* J27, int64_t n, dou
   return SEQVP_OK;
 }
 
+template <bool FREE>
+static int jd_root_launch(const double* u, const double* v, const double* k, double limit, const Dims& d, double* out,
+                          void* ws, hipStream_t s) {
+  const RootWorkspace r = root_workspace(d);
+  double* part = (double*)ws;
+  unsigned long long* count = (unsigned long long*)((char*)ws + r.count_offset);
+  int64_t* list = (int64_t*)((char*)ws + r.list_offset);
+  hipLaunchKernelGGL(root_count_zero_kernel, dim3(1), dim3(1), 0, s, count);
+  hipLaunchKernelGGL((jd_root_screen_kernel<FREE>), dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, v, k, limit, d,
+                     count, list);
+  hipLaunchKernelGGL((jd_root_kernel<FREE>), dim3(r.eval_blocks), dim3(BLOCK), 0, s, u, v, k, limit, d, count, list,
+                     part);
+  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, r.eval_blocks, out);
+  return launch_status();
+}
+
 extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, double limit, int64_t nx, int64_t ny,
                              int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
   if (!u || !v || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
-  const Dims d = make_dims(nx, ny, nz);
-  const int64_t nb = cdiv(d.ncell, BLOCK);
-  hipStream_t s = (hipStream_t)stream;
-  double* part = (double*)ws;
-  hipLaunchKernelGGL((jd_root_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, v, k, limit, d, part);
-  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, nb, out);
-  return launch_status();
+  return jd_root_launch<false>(u, v, k, limit, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
 }
 
 extern "C" int seqvp_jd_root_free(const double* u, const double* p, const double* k, double limit, int64_t nx,
                                   int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
   if (!u || !p || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
-  const Dims d = make_dims(nx, ny, nz);
-  const int64_t nb = cdiv(d.ncell, BLOCK);
-  hipStream_t s = (hipStream_t)stream;
-  double* part = (double*)ws;
-  hipLaunchKernelGGL((jd_root_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, p, k, limit, d, part);
-  hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, nb, out);
-  return launch_status();
+  return jd_root_launch<true>(u, p, k, limit, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
 }
 
 extern "C" int seqvp_node_energy(const double* warped, const double* fixed, const double* data_weight,

@@ -38,4 +38,7 @@ def test_host_only_queries():
     # the sine matrices are stored folded in 16-output strips: ceil(n/2) rows rounded up to 16, 16 columns per strip
     # (ceil(ceil(n/2)/16) even strips and ceil(floor(n/2)/16) odd ones)
     assert _lib.workspace_bytes(6, (nz, ny, nx)) == 8 * (3 * 158 * 190 * 222 + 112 * 224 + 96 * 192 + 80 * 160)
+    # the T66 sweep: partials of its cubic pass (at most 512 workgroups), the listed-wave count (256 bytes), one
+    # list slot per wave of 64 cells
+    assert _lib.workspace_bytes(5, (nz, ny, nx)) == 512 * 3 * 8 + 256 + 8 * ((ncell + 63) // 64)
     assert _lib.workspace_bytes(0, (1, 5, 5)) == 0

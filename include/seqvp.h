@@ -217,6 +217,13 @@ int seqvp_jd_root(const double* u, const double* v, const double* k, double limi
 int seqvp_jd_root_free(const double* u, const double* p, const double* k, double limit, int64_t nx, int64_t ny,
                        int64_t nz, double* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* seqvp_jd_root_free with the limit read from device memory (`limit`: one double, written by seqvp_pg_step), so the
+ * sweep can follow the direction without the host reading the step first. A limit that is not positive and finite
+ * gives results the caller must discard (the host's driver stops before such a sweep). Same kernels, same bits. */
+int seqvp_jd_root_free_dev(const double* u, const double* p, const double* k, const double* limit, int64_t nx,
+                           int64_t ny, int64_t nz, double* out, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
 /* T75 Bernstein floating-point prescreen (replaces prescreen_lower_bounds,
  * jd_curl/three_d/certificate_prescreen.py:125-153; kernel :60-121), bitwise in IEEE binary64 with no
  * contraction and the reference operation order. D: closed27 derivative table (3,27,8) (device).
@@ -300,6 +307,20 @@ int seqvp_direction_scalars(const double* theta, const double* theta_old, const 
                             const double* p_old, const double* p, const double* u, double ox, double oy, double oz,
                             double hx, double hy, double hz, int64_t nx, int64_t ny, int64_t nz, double* out, void* ws,
                             size_t ws_bytes, void* stream);
+
+/* The first attempt's step of a PG iteration on the device (the host step of poisson_gradient.py:955-990 and
+ * 1340-1356): from `scalars` as seqvp_direction_scalars leaves them (BB2 products out[4..7], bound out[8]), the BB2
+ * proposal with the reference's fallbacks (has_prev = 0: initial_step; else 2 min(prev_tau, step_max/2) when the
+ * secant is nonfinite, has s.y <= 0, cosine <= 1e-12 or y.iy <= 0, or s.y / y.iy is not positive), clipped to
+ * [step_min, step_max], then min(proposal, 0.9 bound) into tau (one device double). The operations are the host
+ * driver's, in its order, so both give the same bits. */
+int seqvp_pg_step(const double* scalars, int has_prev, double prev_tau, double step_min, double step_max,
+                  double initial_step, double* tau, void* stream);
+
+/* The step capped by the T66 root (poisson_gradient.py:1356-1366): tau = min(limit, safety * root_out[0] * limit)
+ * when root_out (seqvp_jd_root_free's out) holds a finite nonzero root and no nonfinite flag; else tau = limit.
+ * limit and tau: one device double each. */
+int seqvp_capped_step(const double* limit, const double* root_out, double safety, double* tau, void* stream);
 
 /* The same candidate map with the step read from device memory (tau: one double), so a captured trial segment
  * (seqvp_graph_*) takes a new step on every launch; same arithmetic, same bits. Counterpart of the host axpy of

@@ -45,6 +45,9 @@ SIGNATURES = {
     "seqvp_face_interior_minima_host": [c_vp, c_i64, c_vp],
     "seqvp_jd_root": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_jd_root_free": [c_vp, c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_jd_root_free_dev": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_pg_step": [c_vp, c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp],
+    "seqvp_capped_step": [c_vp, c_vp, c_dbl, c_vp, c_vp],
     "seqvp_prescreen": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_prescreen_flags": [c_vp, c_vp, c_dbl, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp],
     "seqvp_spline_warp": [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_dbl, c_dbl,

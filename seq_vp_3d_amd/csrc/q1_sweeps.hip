@@ -1419,9 +1419,11 @@ __device__ __forceinline__ void load_direction(const double* __restrict__ v, int
 template <bool FREE>
 __global__ void __launch_bounds__(BLOCK, 3) jd_root_screen_kernel(const double* __restrict__ u,
                                                                   const double* __restrict__ v,
-                                                                  const double* __restrict__ k, double L, Dims d,
+                                                                  const double* __restrict__ k, double L_arg,
+                                                                  const double* __restrict__ L_dev, Dims d,
                                                                   unsigned long long* __restrict__ count,
                                                                   int64_t* __restrict__ list) {
+  const double L = L_dev ? *L_dev : L_arg;
   const int64_t c = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
   bool skip = true;
   if (c < d.ncell) {
@@ -1452,10 +1454,12 @@ __global__ void __launch_bounds__(BLOCK, 3) jd_root_screen_kernel(const double* 
 // torch_ops.py:636-673).
 template <bool FREE>
 __global__ void __launch_bounds__(BLOCK) jd_root_kernel(const double* __restrict__ u, const double* __restrict__ v,
-                                                        const double* __restrict__ k, double L, Dims d,
+                                                        const double* __restrict__ k, double L_arg,
+                                                        const double* __restrict__ L_dev, Dims d,
                                                         const unsigned long long* __restrict__ count,
                                                         const int64_t* __restrict__ list,
                                                         double* __restrict__ partials) {
+  const double L = L_dev ? *L_dev : L_arg;
   double best = __builtin_inf();
   double invalid = 0.0, bad = 0.0;
   const int64_t listed = (int64_t)*count;
@@ -2126,17 +2130,17 @@ extern "C" int seqvp_face_interior_minima_host(const double* J27, int64_t n, dou
 }
 
 template <bool FREE>
-static int jd_root_launch(const double* u, const double* v, const double* k, double limit, const Dims& d, double* out,
-                          void* ws, hipStream_t s) {
+static int jd_root_launch(const double* u, const double* v, const double* k, double limit, const double* limit_dev,
+                          const Dims& d, double* out, void* ws, hipStream_t s) {
   const RootWorkspace r = root_workspace(d);
   double* part = (double*)ws;
   unsigned long long* count = (unsigned long long*)((char*)ws + r.count_offset);
   int64_t* list = (int64_t*)((char*)ws + r.list_offset);
   hipLaunchKernelGGL(root_count_zero_kernel, dim3(1), dim3(1), 0, s, count);
-  hipLaunchKernelGGL((jd_root_screen_kernel<FREE>), dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, v, k, limit, d,
-                     count, list);
-  hipLaunchKernelGGL((jd_root_kernel<FREE>), dim3(r.eval_blocks), dim3(BLOCK), 0, s, u, v, k, limit, d, count, list,
-                     part);
+  hipLaunchKernelGGL((jd_root_screen_kernel<FREE>), dim3(cdiv(d.ncell, BLOCK)), dim3(BLOCK), 0, s, u, v, k, limit,
+                     limit_dev, d, count, list);
+  hipLaunchKernelGGL((jd_root_kernel<FREE>), dim3(r.eval_blocks), dim3(BLOCK), 0, s, u, v, k, limit, limit_dev, d,
+                     count, list, part);
   hipLaunchKernelGGL(jd_root_reduce_kernel, dim3(1), dim3(RB), 0, s, part, r.eval_blocks, out);
   return launch_status();
 }
@@ -2145,14 +2149,22 @@ extern "C" int seqvp_jd_root(const double* u, const double* v, const double* k, 
                              int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
   if (!u || !v || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
-  return jd_root_launch<false>(u, v, k, limit, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
+  return jd_root_launch<false>(u, v, k, limit, nullptr, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
 }
 
 extern "C" int seqvp_jd_root_free(const double* u, const double* p, const double* k, double limit, int64_t nx,
                                   int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
   if (!u || !p || !k || !out || !grid_ok(nx, ny, nz) || !(limit > 0.0)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
-  return jd_root_launch<true>(u, p, k, limit, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
+  return jd_root_launch<true>(u, p, k, limit, nullptr, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
+}
+
+extern "C" int seqvp_jd_root_free_dev(const double* u, const double* p, const double* k, const double* limit,
+                                      int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                      void* stream) {
+  if (!u || !p || !k || !limit || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (!ws || ws_bytes < q1_workspace(5, nx, ny, nz)) return SEQVP_EWORKSPACE;
+  return jd_root_launch<true>(u, p, k, 0.0, limit, make_dims(nx, ny, nz), out, ws, (hipStream_t)stream);
 }
 
 extern "C" int seqvp_node_energy(const double* warped, const double* fixed, const double* data_weight,

@@ -415,6 +415,73 @@ extern "C" int seqvp_direction_scalars(const double* theta, const double* theta_
   return launch_status();
 }
 
+// The first attempt's step from the direction scalars (`s` as seqvp_direction_scalars leaves them): the BB2
+// proposal with the reference's fallbacks (poisson_gradient.py:955-990), clipped to [step_min, step_max], then
+// min(proposal, 0.9 bound) (poisson_gradient.py:1340-1356). Written as the driver's host code computes it,
+// operation for operation (IEEE sqrt and division, no contraction), so both give the same bits; min/max follow the
+// host's order of arguments for NaN (fmin/fmax return the number).
+__global__ void pg_step_kernel(const double* __restrict__ s, int has_prev, double prev_tau, double step_min,
+                               double step_max, double initial_step, double* __restrict__ tau) {
+#pragma clang fp contract(off)
+  auto protect = [&](double v) { return fmin(step_max, fmax(step_min, v)); };
+  double proposal;
+  if (!has_prev) {
+    proposal = protect(initial_step);
+  } else {
+    const double fallback = protect(2.0 * fmin(prev_tau, step_max / 2.0));
+    const double sy = s[4], ns = sqrt(s[5]), ny = sqrt(s[6]), yiy = s[7];
+    if (!(isfinite(sy) && isfinite(ns) && isfinite(ny) && isfinite(yiy))) {
+      proposal = fallback;
+    } else if (sy <= 0.0) {
+      proposal = fallback;
+    } else {
+      const double cosine = (ns > 0.0 && ny > 0.0) ? sy / (ns * ny) : 0.0;
+      if (!isfinite(cosine) || cosine <= 1e-12) {
+        proposal = fallback;
+      } else if (yiy <= 0.0) {
+        proposal = fallback;
+      } else {
+        const double q = sy / yiy;
+        proposal = (!isfinite(q) || q <= 0.0) ? fallback : protect(q);
+      }
+    }
+  }
+  const double capped = 0.9 * s[8];
+  tau[0] = capped < proposal ? capped : proposal;
+}
+
+// The trial step capped by the T66 root (poisson_gradient.py:1356-1366): root = out[0] * limit in step units;
+// unchanged when the sweep met a nonfinite coefficient (out[2] != 0), found no root, or the root is 0.
+__global__ void capped_step_kernel(const double* __restrict__ limit, const double* __restrict__ root_out,
+                                   double safety, double* __restrict__ tau) {
+#pragma clang fp contract(off)
+  const double L = limit[0], rs = root_out[0];
+  double t = L;
+  if (root_out[2] == 0.0 && isfinite(rs)) {
+    const double root = rs * L;
+    if (root != 0.0) {
+      const double c = safety * root;
+      t = c < t ? c : t;
+    }
+  }
+  tau[0] = t;
+}
+
+extern "C" int seqvp_pg_step(const double* scalars, int has_prev, double prev_tau, double step_min, double step_max,
+                             double initial_step, double* tau, void* stream) {
+  if (!scalars || !tau) return SEQVP_EINVAL;
+  hipLaunchKernelGGL(pg_step_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, scalars, has_prev, prev_tau,
+                     step_min, step_max, initial_step, tau);
+  return launch_status();
+}
+
+extern "C" int seqvp_capped_step(const double* limit, const double* root_out, double safety, double* tau,
+                                 void* stream) {
+  if (!limit || !root_out || !tau) return SEQVP_EINVAL;
+  hipLaunchKernelGGL(capped_step_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, limit, root_out, safety, tau);
+  return launch_status();
+}
+
 extern "C" int seqvp_dots(const double* const* xs, const double* const* ys, int npairs, int64_t n, double* out,
                           void* ws, size_t ws_bytes, void* stream) {
   if (!xs || !ys || !out || npairs < 1 || npairs > 4 || n < 0) return SEQVP_EINVAL;

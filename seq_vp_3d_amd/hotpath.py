@@ -30,7 +30,7 @@ from .poisson import DevicePoisson, SpectralMetric
 from .certificate import CertificateConfig, DeviceCertificate
 from .graphs import SegmentGraphs, key_of
 from .jd_step import nominate_witness
-from .q1 import OP_TRIAL, FusedJDRoot, FusedQ1
+from .q1 import OP_ROOT, OP_TRIAL, FusedJDRoot, FusedQ1
 from .spline import DeviceSpline
 
 JD_AFFINE_FLOOR = 1e-12
@@ -145,6 +145,12 @@ class PairState:
         self._tau_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
         self._tau_dev = z(1)
         self._trial_dev = z(17)
+        # the chained first attempt: [step, root sweep out (3), capped step] on the device and its host copy
+        self._chain_dev = z(5)
+        self._chain_host = torch.zeros(5, dtype=torch.float64, pin_memory=True)
+        self._chain_np = self._chain_host.numpy()
+        self.chain = True           # chain_launch enqueues (False: it declines, and the driver takes the host path)
+        self.chained_trials = 0     # chained first attempts whose trial the driver used
         # least closed27 J of the accepted state when known (host float and device copy, the copy written before
         # its next use): at or above the activation, the trial sweep needs no closed27 J of the accepted state
         self._base_min, self._base_min_dev, self._base_min_stale, self._trial_min = None, z(1), False, None
@@ -382,18 +388,25 @@ class PairState:
             _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(self._trial_dev),
                       17 * 8, self._stream())
         self.cert._u = self.u_t
+        return self._trial_read()
+
+    def _trial_read(self):
         self._wait()
         vals = self._trial_np.tolist()
         nbad = int(self._trial_np[11:12].view(np.int64)[0])
         self._trial_min = vals[8]
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
 
-    def _trial_segment(self, energies=True, quiet=False):
+    def _trial_segment(self, energies=True, quiet=False, tau_dev=None):
+        """The trial's device work; the step from the page-locked word, or from tau_dev (one device double) when
+        given."""
         nz, ny, nx = self.shape
         q, w, s = self.q1, self.problem.weights, self._stream()
         d = self._trial_dev
-        _lib.call("seqvp_copy_async", _lib.ptr(self._tau_dev), ctypes.c_void_p(self._tau_host.data_ptr()), 8, s)
-        _lib.call("seqvp_axpy_inject_dev", _lib.ptr(self.theta), _lib.ptr(self.p), _lib.ptr(self._tau_dev), nx, ny, nz,
+        if tau_dev is None:
+            tau_dev = self._tau_dev
+            _lib.call("seqvp_copy_async", _lib.ptr(tau_dev), ctypes.c_void_p(self._tau_host.data_ptr()), 8, s)
+        _lib.call("seqvp_axpy_inject_dev", _lib.ptr(self.theta), _lib.ptr(self.p), _lib.ptr(tau_dev), nx, ny, nz,
                   _lib.ptr(self.theta_t), _lib.ptr(self.u_t), s)
         self.cert.screen(self.u_t, summary=d[12:17])
         self.spline.warp(self.u_t, value=self.warped_t, gradient=self.bgrad_t, status=d[11:12].view(torch.int64))
@@ -418,6 +431,57 @@ class PairState:
         vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
         self._trial_min = None if screen else vals[8]      # the FP32 screen's min J is not the closed27 J
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)
+
+    # ---- the chained first attempt ---------------------------------------------------------------------
+    def chain_launch(self, prev_tau, step_min, step_max, initial_step, safety, has_prev=True):
+        """Enqueues, behind the pending direction segment, the next line search's first attempt as the host driver
+        would take it when its witness memory is empty: the step from the direction scalars (seqvp_pg_step), the T66
+        root on that limit, the step capped by the root, and the trial segment on that step. Returns the pending
+        chain (None when this state does not chain: captured segments or a batched group). The driver compares its
+        own step with the chain's (chain_matches, chain_step) before it uses chain_root and chain_trial; a chain it
+        does not use is simply not read (it writes only the trial's buffers)."""
+        if not self.chain or self.graphs is not None or self.batch is not None:
+            return None
+        nz, ny, nx = self.shape
+        s = self._stream()
+        c = self._chain_dev
+        _lib.call("seqvp_pg_step", _lib.ptr(self.scalars), int(has_prev), float(prev_tau), float(step_min),
+                  float(step_max), float(initial_step), _lib.ptr(c[0:1]), s)
+        w, n = self.q1._workspace(OP_ROOT)
+        _lib.call("seqvp_jd_root_free_dev", _lib.ptr(self.u), _lib.ptr(self.p), _lib.ptr(self.q1.k), _lib.ptr(c[0:1]),
+                  nx, ny, nz, _lib.ptr(c[1:4]), w, n, s)
+        _lib.call("seqvp_capped_step", _lib.ptr(c[0:1]), _lib.ptr(c[1:4]), float(safety), _lib.ptr(c[4:5]), s)
+        _lib.call("seqvp_copy_async", ctypes.c_void_p(self._chain_host.data_ptr()), _lib.ptr(c), 5 * 8, s)
+        root_done = torch.cuda.Event()
+        root_done.record()
+        quiet = self._quiet() and self._base_min_ptr() is not None
+        self._trial_segment(True, quiet, tau_dev=c[4:5])
+        return dict(root_done=root_done, u_t=self.u_t, theta=self.theta, p=self.p, version=self._version)
+
+    def chain_matches(self, chain, tau):
+        """Whether the chain's step (waiting for its root sweep) is the host's `tau`, bit for bit, and the chain
+        still belongs to the current accepted state and direction."""
+        chain["root_done"].synchronize()
+        return (chain["version"] == self._version and chain["theta"] is self.theta and chain["p"] is self.p
+                and chain["u_t"] is self.u_t and float(self._chain_np[0]) == tau)
+
+    def chain_root(self, chain):
+        """(root, invalid, points) of the chain's T66 sweep, as jd_bound_read gives them."""
+        root, invalid, bad = (float(v) for v in self._chain_np[1:4])
+        if bad:
+            raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
+        return root, int(invalid), 27 * self.q1.ncell
+
+    def chain_step(self, chain):
+        """The step the chained trial used."""
+        return float(self._chain_np[4])
+
+    def chain_trial(self, chain):
+        """The chained trial's result, as trial(chain_step(chain)) gives it."""
+        self._certificate = None
+        self.cert._u = self.u_t
+        self.chained_trials += 1
+        return self._trial_read()
 
     def _trial_result(self, n, s, b, nbad, summary, screen):
         """The TrialResult of a candidate from its node energies, soft and barrier terms, invalid-sample count and
@@ -566,11 +630,14 @@ class PairState:
                      self.u, self.theta, self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale,
                      self._den, self.q1.ws.buf, self.dot_ws, self.scalars)
         self._segment(key, lambda: self._direction_segment(prev, fp32, sweep, extra))
-        return prev is not None, norm, ratio
+        done = torch.cuda.Event()
+        done.record()
+        return prev is not None, norm, ratio, done
 
     def direction_scalars_read(self, pending):
-        has_prev, norm, ratio = pending
-        self._wait()
+        """The values of a direction_scalars_launch (waits for its segment only, not for work enqueued after it)."""
+        has_prev, norm, ratio, done = pending
+        done.synchronize()
         v = self._scalars_np.tolist()
         bb2 = (v[4], float(np.sqrt(v[5])), float(np.sqrt(v[6])), v[7]) if has_prev else None
         G = float(np.sqrt(v[1])) / float(np.sqrt(self.grid.omega)) if norm else None

@@ -239,6 +239,7 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
 
     ratio_pending = False       # the last accepted step's ratio is on the device, read with the next scalars
     prefetched = None           # the next iteration's direction scalars, enqueued at the acceptance
+    chained = None              # the next iteration's first attempt, enqueued behind them (PairState.chain_launch)
     while True:
         scalars = None
         try:
@@ -302,15 +303,19 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
         if bound <= 0:
             return finish("domain_blocked", "NO_POSITIVE_DOMAIN_STEP")
         tau = min(proposal, .9 * bound)
+        # a chained first attempt is used only when its device step is this step, bit for bit
+        chain, chained = chained, None
+        if chain is not None and not st.chain_matches(chain, tau):
+            chain = None
         # JD step prediction: the historical witnesses, then (T66) the global closed27 root on the same limit
         # (poisson_gradient.py:1356-1366, 708-735); the root's sweep is enqueued first and runs while the host
-        # evaluates the witnesses
-        pending_root = st.jd_bound_launch(tau) if config.jd_step_global else None
+        # evaluates the witnesses (or already ran in the chain)
+        pending_root = st.jd_bound_launch(tau) if config.jd_step_global and chain is None else None
         predicted, prediction = memory.predict(st.witness_coefficients, tau, config.jd_step_safety)
         root_info = None
         if config.jd_step_global:
             try:
-                root_s, invalid, points = st.jd_bound_read(pending_root)
+                root_s, invalid, points = st.chain_root(chain) if chain is not None else st.jd_bound_read(pending_root)
             except FloatingPointError as exc:
                 root_s, invalid, points = float("nan"), 0, 0
                 root_info = dict(fallback=str(exc))
@@ -335,6 +340,8 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
             required = -config.c1 * tau * slope
             if config.fp32_screen and not screen_record(st)["disabled"]:
                 tr = screened_trial(st, tau, required)
+            elif attempt == 0 and chain is not None and st.chain_step(chain) == tau:
+                tr = st.chain_trial(chain)
             else:
                 tr = st.trial(tau)
             if tr.witness is not None:
@@ -382,6 +389,13 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                 if metric_ready and accepted_steps + 1 < config.max_iterations:
                     prefetched = st.direction_scalars_launch(previous, norm=True, ratio=True,
                                                              fp32=config.fp32_direction, gradient=True)
+                    # with an empty witness memory the next first attempt's step depends only on device values:
+                    # it is enqueued too (the host checks it before use)
+                    launch = getattr(st, "chain_launch", None)
+                    if (launch is not None and config.jd_step_global and not config.fp32_screen
+                            and not memory.points):
+                        chained = launch(tau, config.step_min, config.step_max, config.initial_step,
+                                         config.jd_step_safety)
             row = dict(iteration=accepted_steps, attempt=attempt + 1, accepted=bool(accepted), reason=reason, tau=tau,
                        step_proposal=proposal, step_rule=rule, untruncated=untruncated, domain_step_bound=bound,
                        gradient_norm=G, gradient_dot_direction=slope, trial_valid=bool(tr.ok and not tr.unproven_cells),

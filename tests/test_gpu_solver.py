@@ -75,6 +75,46 @@ def test_fp32_screen_keeps_the_fp64_decisions(gpu_device):
     assert rec["uses"] > 0 and rec["checks"] >= 1 and not rec["disabled"] and rec["max_rel"] <= 1.
 
 
+@pytest.mark.parametrize("initial_step", [1., 10.])
+def test_chained_first_attempts_are_the_host_path(gpu_device, initial_step):
+    """With the T66 bound on, PairState enqueues each next line search's first attempt on the device (step from the
+    direction scalars, root, capped step, trial) at the acceptance; the driver uses it only when the device step is
+    its own step bit for bit. Every history row (inner and outer) and the final state equal those of the host path
+    (chain off); with initial step 10 the BB2 fallbacks and the root cap bind more often."""
+    from seq_vp_3d_amd.hotpath import PairState
+    from seq_vp_3d_amd.problem import Q1Problem, native_weights
+    from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
+    from seq_vp_3d_amd.synthetic import PairSpec, make_pair
+    print("case chained first attempts 20x22x24", initial_step, flush=True)
+
+    def sampler(img, grid, pts):
+        s = osp.build(img.cpu().numpy(), Q.Grid(grid.shape, grid.spacing))
+        return torch.tensor(osp.sample(s, pts.cpu().numpy(), with_gradient=False)[0], device=img.device)
+
+    spec = PairSpec((20, 22, 24), 8, (2., 5.), 3, .6, .01, 5)
+    pair = make_pair(spec, "cpu", sampler)
+    w = native_weights(spec.shape)
+    runs = []
+    for chain in (False, True):
+        st = PairState(Q1Problem(pair.grid, w, 0., .05), pair.fixed, pair.moving, gpu_device)
+        st.chain = chain
+        st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=gpu_device))
+        rows, inner = [], []
+        res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=True, initial_step=initial_step),
+                               max_outer=3, observer=rows.append, inner_observer=inner.append)
+        runs.append((res, rows, inner, st))
+    (r0, rows0, inner0, st0), (r1, rows1, inner1, st1) = runs
+    assert st0.chained_trials == 0 and st1.chained_trials > 0
+    assert r0.status == r1.status and r0.outer_accepted == r1.outer_accepted
+    skip = {"seconds", "elapsed_seconds"}
+    assert len(inner0) == len(inner1)
+    for a, b in zip(inner0, inner1):
+        assert {k: v for k, v in a.items() if k not in skip} == {k: v for k, v in b.items() if k not in skip}
+    for a, b in zip(rows0, rows1):
+        assert {k: v for k, v in a.items() if k not in skip} == {k: v for k, v in b.items() if k not in skip}
+    assert torch.equal(st0.theta, st1.theta) and torch.equal(st0.u, st1.u)
+
+
 @pytest.mark.parametrize("jd_step_global", [False, True])
 def test_witness_memory_and_prediction_match_cpu_twin(gpu_device, jd_step_global):
     """A first step far too long folds the map: the device nominates the same confirmed witness, the memory

@@ -433,11 +433,12 @@ class PairState:
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)
 
     # ---- the chained first attempt ---------------------------------------------------------------------
-    def chain_launch(self, prev_tau, step_min, step_max, initial_step, safety, has_prev=True):
+    def chain_launch(self, prev_tau, step_min, step_max, initial_step, safety, has_prev=True, root=True):
         """Enqueues, behind the pending direction segment, the next line search's first attempt as the host driver
-        would take it when its witness memory is empty: the step from the direction scalars (seqvp_pg_step), the T66
-        root on that limit, the step capped by the root, and the trial segment on that step. Returns the pending
-        chain (None when this state does not chain: captured segments or a batched group). The driver compares its
+        would take it when its witness memory is empty: the step from the direction scalars (seqvp_pg_step), with
+        `root` the T66 root on that limit and the step capped by the root, and the trial segment on that step.
+        Returns the pending chain (None when this state does not chain: captured segments or a batched group).
+        The driver compares its
         own step with the chain's (chain_matches, chain_step) before it uses chain_root and chain_trial; a chain it
         does not use is simply not read (it writes only the trial's buffers)."""
         if not self.chain or self.graphs is not None or self.batch is not None:
@@ -447,16 +448,17 @@ class PairState:
         c = self._chain_dev
         _lib.call("seqvp_pg_step", _lib.ptr(self.scalars), int(has_prev), float(prev_tau), float(step_min),
                   float(step_max), float(initial_step), _lib.ptr(c[0:1]), s)
-        w, n = self.q1._workspace(OP_ROOT)
-        _lib.call("seqvp_jd_root_free_dev", _lib.ptr(self.u), _lib.ptr(self.p), _lib.ptr(self.q1.k), _lib.ptr(c[0:1]),
-                  nx, ny, nz, _lib.ptr(c[1:4]), w, n, s)
-        _lib.call("seqvp_capped_step", _lib.ptr(c[0:1]), _lib.ptr(c[1:4]), float(safety), _lib.ptr(c[4:5]), s)
+        if root:
+            w, n = self.q1._workspace(OP_ROOT)
+            _lib.call("seqvp_jd_root_free_dev", _lib.ptr(self.u), _lib.ptr(self.p), _lib.ptr(self.q1.k),
+                      _lib.ptr(c[0:1]), nx, ny, nz, _lib.ptr(c[1:4]), w, n, s)
+            _lib.call("seqvp_capped_step", _lib.ptr(c[0:1]), _lib.ptr(c[1:4]), float(safety), _lib.ptr(c[4:5]), s)
         _lib.call("seqvp_copy_async", ctypes.c_void_p(self._chain_host.data_ptr()), _lib.ptr(c), 5 * 8, s)
         root_done = torch.cuda.Event()
         root_done.record()
         quiet = self._quiet() and self._base_min_ptr() is not None
-        self._trial_segment(True, quiet, tau_dev=c[4:5])
-        return dict(root_done=root_done, u_t=self.u_t, theta=self.theta, p=self.p, version=self._version)
+        self._trial_segment(True, quiet, tau_dev=c[4:5] if root else c[0:1])
+        return dict(root_done=root_done, root=root, u_t=self.u_t, theta=self.theta, p=self.p, version=self._version)
 
     def chain_matches(self, chain, tau):
         """Whether the chain's step (waiting for its root sweep) is the host's `tau`, bit for bit, and the chain
@@ -466,7 +468,8 @@ class PairState:
                 and chain["u_t"] is self.u_t and float(self._chain_np[0]) == tau)
 
     def chain_root(self, chain):
-        """(root, invalid, points) of the chain's T66 sweep, as jd_bound_read gives them."""
+        """(root, invalid, points) of the chain's T66 sweep (a chain launched with root=True), as jd_bound_read gives
+        them."""
         root, invalid, bad = (float(v) for v in self._chain_np[1:4])
         if bad:
             raise FloatingPointError("T66 nonfinite cubic coefficients (fused)")
@@ -474,7 +477,7 @@ class PairState:
 
     def chain_step(self, chain):
         """The step the chained trial used."""
-        return float(self._chain_np[4])
+        return float(self._chain_np[4 if chain["root"] else 0])
 
     def chain_trial(self, chain):
         """The chained trial's result, as trial(chain_step(chain)) gives it."""

@@ -392,10 +392,9 @@ def solve_poisson_gradient(st, target, config=PGConfig(), budget=None, observer=
                     # with an empty witness memory the next first attempt's step depends only on device values:
                     # it is enqueued too (the host checks it before use)
                     launch = getattr(st, "chain_launch", None)
-                    if (launch is not None and config.jd_step_global and not config.fp32_screen
-                            and not memory.points):
+                    if launch is not None and not config.fp32_screen and not memory.points:
                         chained = launch(tau, config.step_min, config.step_max, config.initial_step,
-                                         config.jd_step_safety)
+                                         config.jd_step_safety, root=config.jd_step_global)
             row = dict(iteration=accepted_steps, attempt=attempt + 1, accepted=bool(accepted), reason=reason, tau=tau,
                        step_proposal=proposal, step_rule=rule, untruncated=untruncated, domain_step_bound=bound,
                        gradient_norm=G, gradient_dot_direction=slope, trial_valid=bool(tr.ok and not tr.unproven_cells),

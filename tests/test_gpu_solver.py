@@ -75,12 +75,13 @@ def test_fp32_screen_keeps_the_fp64_decisions(gpu_device):
     assert rec["uses"] > 0 and rec["checks"] >= 1 and not rec["disabled"] and rec["max_rel"] <= 1.
 
 
-@pytest.mark.parametrize("initial_step", [1., 10.])
-def test_chained_first_attempts_are_the_host_path(gpu_device, initial_step):
-    """With the T66 bound on, PairState enqueues each next line search's first attempt on the device (step from the
-    direction scalars, root, capped step, trial) at the acceptance; the driver uses it only when the device step is
-    its own step bit for bit. Every history row (inner and outer) and the final state equal those of the host path
-    (chain off); with initial step 10 the BB2 fallbacks and the root cap bind more often."""
+@pytest.mark.parametrize("initial_step,jd_step_global", [(1., True), (10., True), (1., False)])
+def test_chained_first_attempts_are_the_host_path(gpu_device, initial_step, jd_step_global):
+    """PairState enqueues each next line search's first attempt on the device at the acceptance (step from the
+    direction scalars, with the T66 bound on also the root and the capped step, then the trial); the driver uses it
+    only when the device step is its own step bit for bit. Every history row (inner and outer) and the final state
+    equal those of the host path (chain off); with initial step 10 the BB2 fallbacks and the root cap bind more
+    often."""
     from seq_vp_3d_amd.hotpath import PairState
     from seq_vp_3d_amd.problem import Q1Problem, native_weights
     from seq_vp_3d_amd.solver import PGConfig, SequentialConfig, solve_sequential
@@ -100,7 +101,8 @@ def test_chained_first_attempts_are_the_host_path(gpu_device, initial_step):
         st.chain = chain
         st.set_state(theta=torch.zeros(st.nfree, dtype=torch.float64, device=gpu_device))
         rows, inner = [], []
-        res = solve_sequential(st, SequentialConfig(), PGConfig(jd_step_global=True, initial_step=initial_step),
+        res = solve_sequential(st, SequentialConfig(),
+                               PGConfig(jd_step_global=jd_step_global, initial_step=initial_step),
                                max_outer=3, observer=rows.append, inner_observer=inner.append)
         runs.append((res, rows, inner, st))
     (r0, rows0, inner0, st0), (r1, rows1, inner1, st1) = runs

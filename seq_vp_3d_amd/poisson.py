@@ -130,7 +130,8 @@ class DevicePoisson:
     def _device(self, a):
         if isinstance(a, torch.Tensor):
             return a.to(device=self.device, dtype=torch.float64).reshape(-1).contiguous()
-        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1), device=self.device)
+        # a read-only host array (the reference freezes its arrays) is copied, not wrapped
+        return torch.tensor(np.asarray(a, dtype=np.float64).reshape(-1), dtype=torch.float64, device=self.device)
 
     def prepare_equilibrated(self, scale, gradient):
         """The operands of apply_equilibrated: (scale, gradient) on the device (host arrays are uploaded)

@@ -41,10 +41,10 @@ EXECUTED_FLOPS_PER_UNIT = {
     "seqvp_q1_barrier_energy": 1620.,
     "seqvp_q1_trial_energy": 8699.,
     "seqvp_q1_trial_energy_base_min": 8015.,
-    "seqvp_q1_gradient": 7542. + 24.,
-    "seqvp_q1_gradient_free": 7542. + 24.,
-    "seqvp_q1_ggn_diagonal": 12529.,
-    "seqvp_jd_root": 3646.,
+    "seqvp_q1_gradient": 7129. + 24.,
+    "seqvp_q1_gradient_free": 7129. + 38.,
+    "seqvp_q1_ggn_diagonal": 12127.,
+    "seqvp_jd_root": 532. + 38.,
     "seqvp_spline_warp": 1015.,
 }
 FP64_PEAK_TFLOPS = 78.6
@@ -62,10 +62,11 @@ TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_gradient": 3666e6,
     "seqvp_q1_gradient_free": 3940e6,
     "seqvp_q1_joint_free": 5973e6,
-    "seqvp_jd_root": 361e6,
+    "seqvp_jd_root": 372e6,
+    "seqvp_jd_root_free": 372e6,
     "seqvp_certificate_screen": 283e6,
-    "seqvp_spline_warp": 605e6,
-    "seqvp_dst_poisson": 2540e6,
+    "seqvp_spline_warp": 621e6,
+    "seqvp_dst_poisson": 2535e6,
 }
 
 

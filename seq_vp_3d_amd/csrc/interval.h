@@ -50,6 +50,18 @@ __host__ __device__ __forceinline__ double next_up_finite(double x) {
 }
 __host__ __device__ __forceinline__ double next_down_finite(double x) { return -next_up_finite(-x); }
 
+// 1/h when h is a positive power of two 2^k whose reciprocal is a normal number (|k| < 1022), else 0. For such h the
+// quotient x / h and the product x * 2^-k are roundings of the same real number, so they are the same double for
+// every x (IEEE division and multiplication are both correctly rounded): dividing by the spacing then costs one
+// multiply instead of a division sequence, with identical bits.
+__host__ __device__ __forceinline__ double pow2_reciprocal(double h) {
+  const uint64_t b = bits_of(h);
+  const uint64_t e = (b >> 52) & 0x7ff;
+  if ((b >> 63) || (b & 0x000fffffffffffffull) || e < 2 || e > 2044) return 0.0;
+  return from_bits((uint64_t)(2046 - e) << 52);
+}
+__host__ __device__ __forceinline__ double div_by(double x, double h, double inv) { return inv != 0.0 ? x * inv : x / h; }
+
 // NumPy minimum/maximum: NaN propagates; otherwise the comparison result (ties return the second operand).
 __host__ __device__ __forceinline__ double np_min(double a, double b) {
   if (is_nan(a)) return a;
@@ -158,6 +170,7 @@ template <bool FAST = false>
 __host__ __device__ inline double edge_hull_margin(const double uv[8][3], const double h[3], double delta,
                                                    bool& ok) {
   using O = Ops<FAST>;
+  const double inv[3] = {pow2_reciprocal(h[0]), pow2_reciprocal(h[1]), pow2_reciprocal(h[2])};
   I F[3][3];
   for (int j = 0; j < 3; ++j) {
     for (int r = 0; r < 3; ++r) {
@@ -172,7 +185,7 @@ __host__ __device__ inline double edge_hull_margin(const double uv[8][3], const 
         upper = O::mx(upper, d);
       }
       I e = O::out(lower, upper, good);
-      e = O::out(e.lo / h[j], e.hi / h[j], good);
+      e = O::out(div_by(e.lo, h[j], inv[j]), div_by(e.hi, h[j], inv[j]), good);
       const double id = r == j ? 1.0 : 0.0;
       F[r][j] = O::add(e, point(id), good);
       ok = ok && good;

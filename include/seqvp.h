@@ -48,19 +48,22 @@ int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base,
 /* The trial's soft and barrier energies in one sweep (replaces the pair FusedQ1.soft_energy + barrier_energy of
  * AcceleratedHybridBackend._energy, accelerated.py:1574-1664): out[0:3] = seqvp_q1_soft_energy's [jd, curl,
  * difference], out[3:7] = seqvp_q1_barrier_energy's [value, min J, unweighted change, min base J], bit for bit.
- * `out` is 7 device doubles; workspace op 10 (12 doubles per 256-cell block). */
+ * `out` is 7 device doubles; workspace op 10 (12 doubles per 256-cell block and 256 bytes). */
 int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
                           int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                           size_t workspace_bytes, void* stream);
 
 /* seqvp_q1_trial_energy for a base whose least closed27 J is known (the same trial energy call,
  * triton_q1.py:572-609, given what the previous evaluation of that state returned as its min J): `base_min` is one
- * device double holding the base's least closed27 J over the grid, which the caller has checked is at or above the
- * barrier activation k[37], so that no base point is active. The sweep then evaluates no base closed27 J; out[6] is
- * *base_min and every other output is seqvp_q1_trial_energy's bit for bit. If *base_min is below the activation,
- * out[3] is NaN. base_min == NULL, base == NULL or base == ref: exactly seqvp_q1_trial_energy. */
+ * device double holding the base's least closed27 J over the grid (NULL: unknown). quiet != 0 states that the
+ * caller has checked *base_min is at or above the barrier activation k[37], so that no base point is active: the
+ * sweep then evaluates no base closed27 J and out[6] is *base_min (if *base_min is below the activation, out[3] is
+ * NaN). Otherwise the sweep skips a base cell's closed-point J where the cell's edges bound it above both the
+ * activation and *base_min. Every output is seqvp_q1_trial_energy's bit for bit. (Any call also skips the
+ * closed-point J of trial cells whose edges bound it above the activation and above the least J the sweep has
+ * already met.) quiet requires a base other than ref and a base_min. Workspace op 10. */
 int seqvp_q1_trial_energy_base_min(const double* u, const double* ref, const double* base, const double* base_min,
-                                   const double* k, int64_t nx, int64_t ny, int64_t nz, double* out,
+                                   int quiet, const double* k, int64_t nx, int64_t ny, int64_t nz, double* out,
                                    void* workspace, size_t workspace_bytes, void* stream);
 
 /* One pair of a batched trial-energy launch: the arguments of seqvp_q1_trial_energy (base required). */

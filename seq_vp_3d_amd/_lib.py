@@ -19,7 +19,8 @@ SIGNATURES = {
     "seqvp_q1_soft_energy": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_soft_energy_noalloc": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp],
     "seqvp_q1_trial_energy": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
-    "seqvp_q1_trial_energy_base_min": [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
+    "seqvp_q1_trial_energy_base_min": [c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz,
+                                       c_vp],
     "seqvp_q1_trial_energy_batch": [c_vp, c_int, c_int, c_i64, c_vp],
     "seqvp_q1_barrier_energy": [c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],
     "seqvp_q1_soft_energy_f32": [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_sz, c_vp],

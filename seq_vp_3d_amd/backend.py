@@ -175,7 +175,8 @@ class DeviceIO:
         self.device = torch.device(device)
 
     def upload(self, a):
-        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.device)
+        # a copy: the reference freezes its host arrays, and a read-only array cannot back a tensor
+        return torch.tensor(np.asarray(a, dtype=np.float64), dtype=torch.float64, device=self.device)
 
     def download(self, t):
         out = t.detach().cpu().numpy().copy()

@@ -207,6 +207,77 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
   block_reduce_store<3, 0>(s, none, partials + blk * 6);
 }
 
+// Doubles as unsigned keys in the same order (for atomic min/max).
+__device__ __forceinline__ unsigned long long ordered_key(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_ordered_key(unsigned long long key) {
+  return __longlong_as_double((long long)((key >> 63) ? (key & 0x7fffffffffffffffull) : ~key));
+}
+
+// cofactor rows of a plain 3x3 (no identity): R_r = M_{r+1} x M_{r+2} (geometry.py:11-23).
+__device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const int j = (r + 1) % 3, kk = (r + 2) % 3;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int l = (cc + 1) % 3, m = (cc + 2) % 3;
+      A[r][cc] = M[j][l] * M[kk][m] - M[j][m] * M[kk][l];
+    }
+  }
+}
+
+// A lower bound of J = det(I + grad u) over a cell's closed points, from its edges alone. Every point gradient is a
+// blend of the four parallel edges with weights >= 0 summing to 1, so for any centre c_rj, |grad u(p)_rj - c_rj| <=
+// max_k |e_rjk - c_rj| =: s_rj; with c the edges' mean, F(p) = Fc + E(p), Fc = I + c, |E(p)|_F <= S = |s|_F.
+// Expanding det(Fc + E) like a cubic in E, J(p) >= det Fc - |cof Fc| S - |Fc| S^2/sqrt3 - S^3/(3 sqrt3) (Frobenius
+// norms; |cof M|^2 = e2(sigma^2) <= |M|^4/3 and sigma1 sigma2 sigma3 <= (|M|^2/3)^(3/2)). nf: |Fc| + S >= |F(p)|.
+// A nonfinite edge makes the bound or nf NaN or infinite, so every comparison a caller makes with it fails.
+__device__ __forceinline__ double cell_j_lower_bound(const Edges& E, double& nf) {
+  double F[3][3], S2 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double* e = E.e[j][r];
+      const double m = 0.25 * ((e[0] + e[1]) + (e[2] + e[3]));
+      double sp = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sp = fmax(sp, fabs(e[k] - m));
+      F[r][j] = (r == j ? 1.0 : 0.0) + m;
+      S2 = fma(sp, sp, S2);
+    }
+  double A[3][3];
+  cofactor_plain(F, A);
+  const double det = (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
+  double nf2 = 0.0, na2 = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nf2 = fma(F[r][j], F[r][j], nf2);
+      na2 = fma(A[r][j], A[r][j], na2);
+    }
+  const double inv_sqrt3 = 0.57735026918962584, inv_3sqrt3 = 0.19245008972987526;
+  const double S = sqrt(S2), nfc = sqrt(nf2);
+  nf = nfc + S;
+  return det - (sqrt(na2) * S + nfc * S2 * inv_sqrt3 + S2 * S * inv_3sqrt3);
+}
+
+// cell_j_lower_bound less a margin (1e-12 (1 + nf^3)) that covers the rounding of the bound and of the sweeps' own J:
+// every closed-point J a sweep computes for the cell is at least this value (NaN for a nonfinite edge).
+__device__ __forceinline__ double cell_j_floor(const Edges& E) {
+  double nf;
+  const double jlb = cell_j_lower_bound(E, nf);
+  return jlb - 1e-12 * (1.0 + nf * nf * nf);
+}
+
+// Whether every closed point of the cell has J >= level: a cell that passes has no point below the barrier
+// activation, so its barrier terms (value, change, gradient, curvature) are exactly zero.
+__device__ __forceinline__ bool cell_j_above(const Edges& E, double level) { return cell_j_floor(E) > level; }
+
 // Closed27 barrier terms of one cell from its edges (and its base's): the minima of J (and base J) over the 27 points
 // into m, the barrier value into s[0] and, against a base, the unweighted stable change into s[1]. QUIET: the
 // caller knows the base's least closed27 J over the whole grid, base_min, and that it is >= act, so no base point is
@@ -214,9 +285,18 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
 // value (the same bits: the base's J are the same function of the same edges wherever they are computed).
 template <bool WITH_BASE, bool QUIET = false>
 __device__ __forceinline__ void barrier_cell(Edges& Eu, Edges& Eb, const double* __restrict__ k, DD s[2], double m[2],
-                                             double base_min = 0.0) {
+                                             double base_min = 0.0, bool clear = false) {
   static_assert(!QUIET || WITH_BASE, "a quiet base needs a base");
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
+  // clear: the caller has shown from the edges that no closed point of the cell (nor of its base) is active and
+  // that none can hold a minimum the sweep reports; a wave of clear cells adds nothing.
+  if (__all(clear)) {
+    if constexpr (QUIET) {
+      m[1] = base_min;
+      if (!(base_min >= act)) dd_add(s[0], __builtin_nan(""));
+    }
+    return;
+  }
   // Minima over the 27 points, each with its compile-time closed blend, and whether any point is active.
   bool active = false;
   if constexpr (QUIET) {
@@ -323,8 +403,15 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
                                                    const double* __restrict__ k, const Dims& d,
                                                    double* __restrict__ soft_partials,
                                                    double* __restrict__ barrier_partials, double* er_lds,
-                                                   double base_min = 0.0) {
+                                                   double base_min = 0.0,
+                                                   unsigned long long* __restrict__ umin = nullptr) {
   const int64_t c = blk * BLOCK + threadIdx.x;
+  // umin: the least closed27 J of u this launch's waves have computed so far (an ordered key; +inf at the start):
+  // a cell whose J floor lies above it holds no minimum of u (nor does a base cell whose floor lies above the
+  // base's known least J, base_min)
+  const double ubound = umin ? from_ordered_key(__hip_atomic_load(umin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                             : -__builtin_inf();
+  bool clear = false;
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   DD sb[2] = {{0.0, 0.0}, {0.0, 0.0}};
   double none[1] = {0.0};
@@ -345,6 +432,15 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
       if (WITH_BASE) {
         load_cell(base, b, d, v);
         make_edges(v, ih, Eb);
+      }
+    }
+    if (umin) {
+      const double act = k[K_ACT];
+      const double fu = cell_j_floor(Eu);
+      clear = fu > act && fu > ubound;
+      if (WITH_BASE && !QUIET) {
+        const double fb = cell_j_floor(Eb);
+        clear = clear && fb > act && fb > base_min;   // base_min NaN (unknown): never clear
       }
     }
     double cj = 0.0, cc = 0.0;
@@ -392,7 +488,11 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
     }
     dd_add(s[0], 0.5 * mu * cj);
     dd_add(s[1], 0.5 * nu * cc);
-    barrier_cell<WITH_BASE, QUIET>(Eu, Eb, k, sb, m, base_min);
+    barrier_cell<WITH_BASE, QUIET>(Eu, Eb, k, sb, m, base_min, clear);
+  }
+  if (umin) {
+    const double wm = wave_nan_min(m[0]);
+    if ((threadIdx.x & 63) == 0 && wm < __builtin_inf()) atomicMin(umin, ordered_key(wm));   // NaN: no update
   }
   block_reduce_store<3, 0>(s, none, soft_partials + blk * 6);
   block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
@@ -405,12 +505,15 @@ __global__ void __launch_bounds__(BLOCK, 2) trial_energy_kernel(const double* __
                                                              const double* __restrict__ k, Dims d,
                                                              double* __restrict__ soft_partials,
                                                              double* __restrict__ barrier_partials,
-                                                             const double* __restrict__ base_min) {
+                                                             const double* __restrict__ base_min,
+                                                             unsigned long long* __restrict__ umin) {
   __shared__ double er_lds[36 * BLOCK];
   trial_energy_block<WITH_BASE, BASE_IS_REF, QUIET>(xcd_block(blockIdx.x, gridDim.x), u, ref, base, k, d,
                                                     soft_partials, barrier_partials, er_lds,
-                                                    QUIET ? *base_min : 0.0);
+                                                    base_min ? *base_min : __builtin_nan(""), umin);
 }
+
+__global__ void umin_init_kernel(unsigned long long* __restrict__ umin) { *umin = ordered_key(__builtin_inf()); }
 
 // ---- batched trial energies: many pairs in one persistent launch ------------------------------------------------
 // The work list is every pair's 256-cell blocks, `tile` consecutive blocks per work item; pair q's items start at
@@ -806,65 +909,6 @@ __device__ __forceinline__ void soft_vjp(const Edges& Eu, const lds_double* er, 
   }
 }
 
-// cofactor rows of a plain 3x3 (no identity): R_r = M_{r+1} x M_{r+2} (geometry.py:11-23).
-__device__ __forceinline__ void cofactor_plain(const double M[3][3], double A[3][3]) {
-#pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    const int j = (r + 1) % 3, kk = (r + 2) % 3;
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      const int l = (cc + 1) % 3, m = (cc + 2) % 3;
-      A[r][cc] = M[j][l] * M[kk][m] - M[j][m] * M[kk][l];
-    }
-  }
-}
-
-// A lower bound of J = det(I + grad u) over a cell's closed points, from its edges alone. Every point gradient is a
-// blend of the four parallel edges with weights >= 0 summing to 1, so for any centre c_rj, |grad u(p)_rj - c_rj| <=
-// max_k |e_rjk - c_rj| =: s_rj; with c the edges' mean, F(p) = Fc + E(p), Fc = I + c, |E(p)|_F <= S = |s|_F.
-// Expanding det(Fc + E) like a cubic in E, J(p) >= det Fc - |cof Fc| S - |Fc| S^2/sqrt3 - S^3/(3 sqrt3) (Frobenius
-// norms; |cof M|^2 = e2(sigma^2) <= |M|^4/3 and sigma1 sigma2 sigma3 <= (|M|^2/3)^(3/2)). nf: |Fc| + S >= |F(p)|.
-// A nonfinite edge makes the bound or nf NaN or infinite, so every comparison a caller makes with it fails.
-__device__ __forceinline__ double cell_j_lower_bound(const Edges& E, double& nf) {
-  double F[3][3], S2 = 0.0;
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const double* e = E.e[j][r];
-      const double m = 0.25 * ((e[0] + e[1]) + (e[2] + e[3]));
-      double sp = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) sp = fmax(sp, fabs(e[k] - m));
-      F[r][j] = (r == j ? 1.0 : 0.0) + m;
-      S2 = fma(sp, sp, S2);
-    }
-  double A[3][3];
-  cofactor_plain(F, A);
-  const double det = (F[0][0] * A[0][0] + F[0][1] * A[0][1]) + F[0][2] * A[0][2];
-  double nf2 = 0.0, na2 = 0.0;
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      nf2 = fma(F[r][j], F[r][j], nf2);
-      na2 = fma(A[r][j], A[r][j], na2);
-    }
-  const double inv_sqrt3 = 0.57735026918962584, inv_3sqrt3 = 0.19245008972987526;
-  const double S = sqrt(S2), nfc = sqrt(nf2);
-  nf = nfc + S;
-  return det - (sqrt(na2) * S + nfc * S2 * inv_sqrt3 + S2 * S * inv_3sqrt3);
-}
-
-// Whether every closed point of the cell has J >= level, with a margin (1e-12 (1 + nf^3)) that covers the rounding
-// of the bound and of the sweeps' own J: a cell that passes has no point below the barrier activation, so its
-// barrier terms (value, change, gradient, curvature) are exactly zero.
-__device__ __forceinline__ bool cell_j_above(const Edges& E, double level) {
-  double nf;
-  const double jlb = cell_j_lower_bound(E, nf);
-  return (jlb - level) > 1e-12 * (1.0 + nf * nf * nf);
-}
-
 // quiet: the cell's edges bound J at or above the activation at every closed point (cell_j_above).
 __device__ __forceinline__ void barrier_vjp(Edges& Eu, const double* __restrict__ k, EdgeAdj& G, bool quiet) {
   const double w = k[K_W], delta = k[K_DELTA], act = k[K_ACT], scale = k[K_SCALE];
@@ -1127,14 +1171,6 @@ __global__ void __launch_bounds__(BLOCK) cell_min_jd_kernel(const double* __rest
 }
 
 // Order-preserving map of doubles to unsigned keys (for an integer atomic min).
-__device__ __forceinline__ unsigned long long ordered_key(double x) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-__device__ __forceinline__ double from_ordered_key(unsigned long long key) {
-  return __longlong_as_double((long long)((key >> 63) ? (key & 0x7fffffffffffffffull) : ~key));
-}
-
 __global__ void jd_extension_init_kernel(unsigned long long* __restrict__ counters) {
   if (threadIdx.x < 3) counters[threadIdx.x] = 0ull;
   if (threadIdx.x == 3) counters[3] = ordered_key(__builtin_inf());
@@ -1787,7 +1823,7 @@ static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
     case 5: return root_workspace(d).bytes;
     case 8: return (size_t)d.ncell * sizeof(double);
     case 9: return (size_t)1024 * 8 * sizeof(double);
-    case 10: return (size_t)nblocks * 12 * sizeof(double);
+    case 10: return (size_t)nblocks * 12 * sizeof(double) + 256;   // + the running least J of the sweep
     default: return 0;
   }
 }
@@ -1814,27 +1850,32 @@ extern "C" int seqvp_q1_soft_energy(const double* u, const double* ref, const do
 }
 
 extern "C" int seqvp_q1_trial_energy_base_min(const double* u, const double* ref, const double* base,
-                                              const double* base_min, const double* k, int64_t nx, int64_t ny,
-                                              int64_t nz, double* out, void* ws, size_t ws_bytes, void* stream) {
+                                              const double* base_min, int quiet, const double* k, int64_t nx,
+                                              int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
+                                              void* stream) {
   if (!u || !ref || !k || !out || !grid_ok(nx, ny, nz)) return SEQVP_EINVAL;
+  if (quiet && (!base || !base_min || base == ref)) return SEQVP_EINVAL;
   if (!ws || ws_bytes < q1_workspace(10, nx, ny, nz)) return SEQVP_EWORKSPACE;
   const Dims d = make_dims(nx, ny, nz);
   const int64_t nb = cdiv(d.ncell, BLOCK);
   hipStream_t s = (hipStream_t)stream;
   double* soft = (double*)ws;
   double* bar = soft + nb * 6;
+  unsigned long long* umin = (unsigned long long*)(bar + nb * 6);
+  const double* bm = base ? base_min : nullptr;
+  hipLaunchKernelGGL(umin_init_kernel, dim3(1), dim3(1), 0, s, umin);
   if (base == ref)      // the base is the reference: its Gauss-point terms are the reference's
     hipLaunchKernelGGL((trial_energy_kernel<true, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
-                       (const double*)nullptr);
-  else if (base && base_min)   // a base with no active closed27 point (the caller's guarantee)
+                       bm, umin);
+  else if (quiet)       // a base with no active closed27 point (the caller's guarantee)
     hipLaunchKernelGGL((trial_energy_kernel<true, false, true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d,
-                       soft, bar, base_min);
+                       soft, bar, bm, umin);
   else if (base)
-    hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar,
-                       (const double*)nullptr);
+    hipLaunchKernelGGL((trial_energy_kernel<true>), dim3(nb), dim3(BLOCK), 0, s, u, ref, base, k, d, soft, bar, bm,
+                       umin);
   else
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
-                       (const double*)nullptr);
+                       (const double*)nullptr, umin);
   hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, soft, nb, out);
   hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, bar, nb, out + 3);
   return launch_status();
@@ -1843,7 +1884,7 @@ extern "C" int seqvp_q1_trial_energy_base_min(const double* u, const double* ref
 extern "C" int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
                                      int64_t nx, int64_t ny, int64_t nz, double* out, void* ws, size_t ws_bytes,
                                      void* stream) {
-  return seqvp_q1_trial_energy_base_min(u, ref, base, nullptr, k, nx, ny, nz, out, ws, ws_bytes, stream);
+  return seqvp_q1_trial_energy_base_min(u, ref, base, nullptr, 0, k, nx, ny, nz, out, ws, ws_bytes, stream);
 }
 
 extern "C" int seqvp_q1_trial_energy_batch(const SeqvpTrialBatchEntry* pairs, int npairs, int tile, int64_t grid,

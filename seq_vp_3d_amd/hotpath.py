@@ -154,6 +154,7 @@ class PairState:
         # least closed27 J of the accepted state when known (host float and device copy, the copy written before
         # its next use): at or above the activation, the trial sweep needs no closed27 J of the accepted state
         self._base_min, self._base_min_dev, self._base_min_stale, self._trial_min = None, z(1), False, None
+        self._ref_min, self._ref_min_dev = None, z(1)   # the same for the reference, when it was set to a state
         self._trial_host = torch.zeros(17, dtype=torch.float64, pin_memory=True)
         self._scalars_host = torch.zeros(12, dtype=torch.float64, pin_memory=True)
         self._tau_np, self._trial_np, self._scalars_np = (t.numpy() for t in (self._tau_host, self._trial_host,
@@ -209,6 +210,7 @@ class PairState:
         if ref is not None:
             self.ref.copy_(torch.as_tensor(ref, dtype=torch.float64))
             self._ref_at = None
+            self._ref_min = None
         if theta is not None:
             self.theta.copy_(torch.as_tensor(theta, dtype=torch.float64))
         self.inject(self.theta, self.u)
@@ -377,11 +379,12 @@ class PairState:
         # needs no second warp at promotion), the energies, and one copy of the 17 scalars to the host
         self._tau_np[0] = tau
         batched = self.batch is not None
-        quiet = self._quiet() and self._base_min_ptr() is not None
-        key = key_of("trial", batched, quiet, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
+        known = self._base_min_ptr() is not None     # writes the device copy now, outside any captured segment
+        quiet = self._quiet()
+        key = key_of("trial", batched, known, quiet, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
                      self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
                      self.cert.capacity)
-        self._segment(key, lambda: self._trial_segment(not batched, quiet))
+        self._segment(key, lambda: self._trial_segment(not batched, known, quiet))
         if batched:
             # the soft and barrier sweep in the group's multi-pair launch, then the scalars' copy
             self.batch.trial_energy(self, self._trial_dev[4:11])
@@ -397,7 +400,7 @@ class PairState:
         self._trial_min = vals[8]
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], nbad, vals[12:17], False)
 
-    def _trial_segment(self, energies=True, quiet=False, tau_dev=None):
+    def _trial_segment(self, energies=True, known=False, quiet=False, tau_dev=None):
         """The trial's device work; the step from the page-locked word, or from tau_dev (one device double) when
         given."""
         nz, ny, nx = self.shape
@@ -414,7 +417,7 @@ class PairState:
                       base_warped=self.warped, base_u=self.u, out=d[0:4])
         if energies:
             q.trial_energy(self.u_t, self.ref, self.u, out=d[4:11],      # soft and barrier terms in one sweep
-                           base_min=self._base_min_dev if quiet else None)
+                           base_min=self._base_min_dev if known else None, quiet=quiet)
             _lib.call("seqvp_copy_async", ctypes.c_void_p(self._trial_host.data_ptr()), _lib.ptr(d), 17 * 8, s)
 
     def trial_energies(self, screen=False):
@@ -427,7 +430,7 @@ class PairState:
             sb = torch.cat((q.soft_energy_f32(self.u_t, self.ref, self.u), q.barrier_energy_f32(self.u_t, self.u)))
         else:
             sb = q.trial_energy(self.u_t, self.ref, self.u,       # soft and barrier terms in one sweep
-                                base_min=self._base_min_ptr() if self._quiet() else None)
+                                base_min=self._base_min_ptr(), quiet=self._quiet())
         vals = torch.cat((node, sb, self._trial_bad.to(torch.float64), self.cert.summary)).tolist()
         self._trial_min = None if screen else vals[8]      # the FP32 screen's min J is not the closed27 J
         return self._trial_result(vals[0:4], vals[4:7], vals[7:11], vals[11], vals[12:17], screen)
@@ -456,8 +459,8 @@ class PairState:
         _lib.call("seqvp_copy_async", ctypes.c_void_p(self._chain_host.data_ptr()), _lib.ptr(c), 5 * 8, s)
         root_done = torch.cuda.Event()
         root_done.record()
-        quiet = self._quiet() and self._base_min_ptr() is not None
-        self._trial_segment(True, quiet, tau_dev=c[4:5] if root else c[0:1])
+        known = self._base_min_ptr() is not None
+        self._trial_segment(True, known, self._quiet(), tau_dev=c[4:5] if root else c[0:1])
         return dict(root_done=root_done, root=root, u_t=self.u_t, theta=self.theta, p=self.p, version=self._version)
 
     def chain_matches(self, chain, tau):
@@ -571,7 +574,8 @@ class PairState:
         # a snapshot of the state the reference was set to (the outer start after a refresh) is the reference
         # itself: passed as such, the sweep takes the base's point terms from the reference's
         same = snap.get("version") is not None and snap["version"] == self._ref_at
-        sb = q.trial_energy(self.u, self.ref, self.ref if same else snap["u"]).tolist()
+        sb = q.trial_energy(self.u, self.ref, self.ref if same else snap["u"],
+                            base_min=self._ref_min_dev if same and self._ref_min is not None else None).tolist()
         n, s_, b = node.tolist(), sb[0:3], sb[3:7]
         change = q.barrier_weight * b[2]
         return n[2] + n[3] + s_[2] + change, n[2] + change, min(b[1], b[3]) - q.delta
@@ -593,6 +597,9 @@ class PairState:
         state are then exactly zero; image and barrier do not depend on the reference."""
         self.ref.copy_(self.u)
         self._ref_at = self._version
+        self._ref_min = self._base_min
+        if self._ref_min is not None:
+            self._ref_min_dev.copy_(self._base_min_ptr())
         if self.energy is None:
             self.energy = self.evaluate_energy(self.u, self.warped)
         self.energy = Energies(self.energy.image, 0., 0., 0., self.energy.barrier)

@@ -66,20 +66,20 @@ class FusedQ1:
                   _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         return out
 
-    def trial_energy(self, u, ref, base=None, out=None, base_min=None):
+    def trial_energy(self, u, ref, base=None, out=None, base_min=None, quiet=False):
         """soft_energy(u, ref, base) and barrier_energy(u, base) from one sweep: [jd, curl, difference, barrier
         value, min J, unweighted barrier change, min base J], the same bits as the two calls. base_min: a device
-        double holding the base's least closed27 J, given only when it is at or above the activation (the sweep
-        then skips the base's closed27 J)."""
+        double holding the base's least closed27 J when known; quiet: it is at or above the activation (the sweep
+        then skips the base's closed27 J everywhere; otherwise where the base cell's edges bound it above both)."""
         self._check(u, ref) if base is None else self._check(u, ref, base)
         out = torch.empty(7, dtype=torch.float64, device=self.device) if out is None else out
         w, n = self._workspace(OP_TRIAL)
-        if base_min is None:
+        if base_min is None or base is None:
             _lib.call("seqvp_q1_trial_energy", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base), _lib.ptr(self.k),
                       *self._dims(), _lib.ptr(out), w, n, _lib.stream_handle(self.device))
         else:
             _lib.call("seqvp_q1_trial_energy_base_min", _lib.ptr(u), _lib.ptr(ref), _lib.ptr(base),
-                      _lib.ptr(base_min), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
+                      _lib.ptr(base_min), int(bool(quiet)), _lib.ptr(self.k), *self._dims(), _lib.ptr(out), w, n,
                       _lib.stream_handle(self.device))
         return out
 

@@ -96,9 +96,49 @@ def test_trial_energy_with_a_quiet_base(gpu_device, activation):
     if activation == .7:
         assert float(full[3]) > 0 and float(full[4]) < q.activation
     for _ in range(2):
-        assert torch.equal(q.trial_energy(U, R, B, base_min=bmin), full)
+        assert torch.equal(q.trial_energy(U, R, B, base_min=bmin, quiet=True), full)
+        assert torch.equal(q.trial_energy(U, R, B, base_min=bmin), full)      # known, not declared quiet
     low = torch.full((1,), q.activation / 2, dtype=torch.float64, device=gpu_device)
-    assert torch.isnan(q.trial_energy(U, R, B, base_min=low)[3])
+    assert torch.isnan(q.trial_energy(U, R, B, base_min=low, quiet=True)[3])
+
+
+@pytest.mark.parametrize("activation", [.05, .5])
+def test_trial_energy_skips_are_the_full_sweep(gpu_device, activation):
+    """On a grid of 1,400 workgroups (more than the GPU holds at once, so later waves meet the running least J of
+    earlier ones) with a fold-like dip placed late in the cell order: the trial sweep, which skips the closed-point
+    J of cells whose edges bound it above the activation and above the least J met so far (and of base cells
+    above the base's known least J), gives the separate soft and barrier sweeps' bits, for a general base, a quiet
+    one, the reference as base, and no base."""
+    from seq_vp_3d_amd.q1 import FusedQ1
+    print("case trial sweep skips 64x72x80", activation, flush=True)
+    prob, og, m = case(activation, shape=(64, 72, 80), spacing=(1., 1., 1.))
+    shape = og.shape
+    ref = smooth_field(shape, 1., 5)
+    base = ref + smooth_field(shape, 2., 6)
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
+    dip = np.exp(-((z - .8 * nz) ** 2 + (y - ny / 2) ** 2 + (x - nx / 2) ** 2) / 6.)
+    u = base + smooth_field(shape, 1., 7) + 2.2 * dip[..., None] * np.array([0., 0., -1.]) * np.sign(x - nx / 2)[..., None]
+    for f in (ref, base, u):
+        f[[0, -1]] = 0.
+        f[:, [0, -1]] = 0.
+        f[:, :, [0, -1]] = 0.
+    q = FusedQ1(prob, gpu_device)
+    U, R, B = dev(u, gpu_device), dev(ref, gpu_device), dev(base, gpu_device)
+    want = torch.cat((q.soft_energy(U, R, B), q.barrier_energy(U, B)))
+    assert float(want[4]) < float(want[6])            # the dip lowers the trial's least J
+    for _ in range(2):
+        assert torch.equal(q.trial_energy(U, R, B), want)
+    bmin = q.trial_energy(B, R)[4:5].clone()
+    assert torch.equal(q.trial_energy(U, R, B, base_min=bmin), want)
+    if float(bmin) >= q.activation:
+        assert torch.equal(q.trial_energy(U, R, B, base_min=bmin, quiet=True), want)
+    want_ref = torch.cat((q.soft_energy(U, R, R), q.barrier_energy(U, R)))
+    rmin = q.trial_energy(R, R)[4:5].clone()
+    assert torch.equal(q.trial_energy(U, R, R), want_ref)
+    assert torch.equal(q.trial_energy(U, R, R, base_min=rmin), want_ref)
+    want0 = torch.cat((q.soft_energy(U, R), q.barrier_energy(U)))
+    assert torch.equal(q.trial_energy(U, R), want0)
 
 
 def test_energy_nan_propagates_to_min_j(gpu_device):

@@ -214,12 +214,29 @@ def ptr(t):
 
 
 def stream_handle(device=None):
-    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    """The current HIP stream of `device` (torch's current stream: a caller's `torch.cuda.stream` context applies),
+    read as the raw handle without building a Stream object (a few microseconds per call on the hot path)."""
+    if isinstance(device, torch.device) and device.index is not None:
+        index = device.index
+    elif isinstance(device, int):
+        index = device
+    else:
+        index = torch.cuda.current_device() if device is None else torch.device(device).index
+        if index is None:
+            index = torch.cuda.current_device()
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(index))
+
+
+_WS_BYTES = {}
 
 
 def workspace_bytes(op, shape_zyx):
-    nz, ny, nx = shape_zyx
-    return int(load().seqvp_workspace_bytes(op, nx, ny, nz))
+    key = (op, tuple(shape_zyx))
+    n = _WS_BYTES.get(key)
+    if n is None:
+        nz, ny, nx = shape_zyx
+        n = _WS_BYTES[key] = int(load().seqvp_workspace_bytes(op, nx, ny, nz))
+    return n
 
 
 class Workspace:

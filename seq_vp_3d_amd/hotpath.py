@@ -381,9 +381,10 @@ class PairState:
         batched = self.batch is not None
         known = self._base_min_ptr() is not None     # writes the device copy now, outside any captured segment
         quiet = self._quiet()
-        key = key_of("trial", batched, known, quiet, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
-                     self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
-                     self.cert.capacity)
+        key = None if self.graphs is None else key_of(
+            "trial", batched, known, quiet, self.theta, self.p, self.u, self.warped, self.ref, self.u_t, self.theta_t,
+            self.warped_t, self.bgrad_t, self.fixed, self.data_weight, self.q1.ws.buf, self.cert.undecided,
+            self.cert.capacity)
         self._segment(key, lambda: self._trial_segment(not batched, known, quiet))
         if batched:
             # the soft and barrier sweep in the group's multi-pair launch, then the scalars' copy
@@ -636,9 +637,10 @@ class PairState:
         extra = None if pair is None or (prev is not None and pair[0] is self.theta and pair[1] is prev["theta"]) \
             else pair
         self._ratio_pair = None
-        key = key_of("direction", sweep, fp32, *(prev_key or (None,)), *(extra or (None,)), self.g, self.p,
-                     self.u, self.theta, self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale,
-                     self._den, self.q1.ws.buf, self.dot_ws, self.scalars)
+        key = None if self.graphs is None else key_of(
+            "direction", sweep, fp32, *(prev_key or (None,)), *(extra or (None,)), self.g, self.p, self.u, self.theta,
+            self.warped, self.bgrad, self.ref, self.fixed, self.data_weight, self.scale, self._den, self.q1.ws.buf,
+            self.dot_ws, self.scalars)
         self._segment(key, lambda: self._direction_segment(prev, fp32, sweep, extra))
         done = torch.cuda.Event()
         done.record()

@@ -1028,16 +1028,34 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
     make_edges(v, ih, Eu);
   }
   const bool quiet = cell_j_above(Eu, act);
+  // The curl part nu sum_Gauss qw sum_{j != r} dN_a,j^2 does not depend on u: 24 workgroup constants, one lane each
+  // (every lane of the workgroup reaches the barrier)
+  __shared__ double curl_part[24];
+  if (threadIdx.x < 24) {
+    const int a = threadIdx.x / 3, r = threadIdx.x % 3;
+    double t = 0.0;
+    for (int p = 0; p < 27; ++p) {
+      double sp[3], dn[3];
+      gauss_point(p, sp);
+      basis_grad(a, sp, ih, dn);
+      const double sq0 = dn[0] * dn[0], sq1 = dn[1] * dn[1], sq2 = dn[2] * dn[2];
+      const double cs = r == 0 ? sq1 + sq2 : (r == 1 ? sq0 + sq2 : sq0 + sq1);
+      t += k[K_QW + p] * cs;
+    }
+    curl_part[threadIdx.x] = nu * t;
+  }
+  __syncthreads();
   double acc[8][3];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
     for (int r = 0; r < 3; ++r) acc[a][r] = 0.0;
+  // the JD part mu sum_Gauss qw (A_r . dN_a)^2
 #pragma unroll 1
   for (int p = 0; p < 27; ++p) {
     double sp[3];
     gauss_point(p, sp);
-    const double qw = k[K_QW + p];
+    const double qm = k[K_QW + p] * mu;
     double V[3][3], A[3][3];
     point_grad_w(Eu, GAUSS_W.w[p], V);
     cofactor_jacobian(V, A);
@@ -1045,15 +1063,17 @@ __global__ void __launch_bounds__(BLOCK) ggn_locals_kernel(const double* __restr
     for (int a = 0; a < 8; ++a) {
       double dn[3];
       basis_grad(a, sp, ih, dn);
-      const double sq0 = dn[0] * dn[0], sq1 = dn[1] * dn[1], sq2 = dn[2] * dn[2];
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         const double l = (A[r][0] * dn[0] + A[r][1] * dn[1]) + A[r][2] * dn[2];
-        const double cs = r == 0 ? sq1 + sq2 : (r == 1 ? sq0 + sq2 : sq0 + sq1);
-        acc[a][r] += qw * (mu * (l * l) + nu * cs);
+        acc[a][r] = fma(qm * l, l, acc[a][r]);
       }
     }
   }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) acc[a][r] += curl_part[3 * a + r];
   // psi''(J) is exactly 0 at and above the activation: a wave whose cells' edges bound J at or above it at every
   // closed point (cell_j_above) skips the barrier part; else J at the 27 points with their compile-time closed
   // blends first, and a wave with no point below it skips it too.

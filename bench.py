@@ -40,10 +40,10 @@ EXECUTED_FLOPS_PER_UNIT = {
     "seqvp_q1_soft_energy": 7223.,
     "seqvp_q1_barrier_energy": 1620.,
     "seqvp_q1_trial_energy": 8699.,
-    "seqvp_q1_trial_energy_base_min": 8015.,
+    "seqvp_q1_trial_energy_base_min": 7614.,
     "seqvp_q1_gradient": 7129. + 24.,
-    "seqvp_q1_gradient_free": 7129. + 38.,
-    "seqvp_q1_ggn_diagonal": 12127.,
+    "seqvp_q1_gradient_free": 7127. + 38.,
+    "seqvp_q1_ggn_diagonal": 8781. + 38.,
     "seqvp_jd_root": 532. + 38.,
     "seqvp_spline_warp": 1015.,
 }
@@ -62,11 +62,11 @@ TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_gradient": 3666e6,
     "seqvp_q1_gradient_free": 3940e6,
     "seqvp_q1_joint_free": 5973e6,
-    "seqvp_jd_root": 372e6,
-    "seqvp_jd_root_free": 372e6,
+    "seqvp_jd_root": 381e6,
+    "seqvp_jd_root_free": 381e6,
     "seqvp_certificate_screen": 283e6,
-    "seqvp_spline_warp": 621e6,
-    "seqvp_dst_poisson": 2535e6,
+    "seqvp_spline_warp": 610e6,
+    "seqvp_dst_poisson": 2533e6,
 }
 
 

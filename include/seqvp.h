@@ -316,7 +316,7 @@ int seqvp_direction_scalars(const double* theta, const double* theta_old, const 
  * proposal with the reference's fallbacks (has_prev = 0: initial_step; else 2 min(prev_tau, step_max/2) when the
  * secant is nonfinite, has s.y <= 0, cosine <= 1e-12 or y.iy <= 0, or s.y / y.iy is not positive), clipped to
  * [step_min, step_max], then min(proposal, 0.9 bound) into tau (one device double). The operations are the host
- * driver's, in its order, so both give the same bits. */
+ * solver's, in its order, so both give the same bits. */
 int seqvp_pg_step(const double* scalars, int has_prev, double prev_tau, double step_min, double step_max,
                   double initial_step, double* tau, void* stream);
 

@@ -22,10 +22,15 @@ constexpr int BLOCK = 256;
 // native sweep is 26,454 blocks; 1024 threads fold 26 each).
 constexpr int RB = 1024;
 
+// blocks of a one-lane-per-cell sweep over the grid's cells (the partials' component stride)
+__device__ __forceinline__ int64_t cell_blocks(const Dims& d) { return (d.ncell + BLOCK - 1) / BLOCK; }
+
 // ------------------------------------------------------------------------------------------------
-// Block and grid reductions of NS double-double sums and NM NaN-propagating minima.
+// Block and grid reductions of NS double-double sums and NM NaN-propagating minima. A block's 2 NS + NM
+// results go to out[0], out[cs], out[2 cs], ...: the sweeps store their partials component by component
+// (cs = the number of blocks), so that the fold below reads each component with coalesced loads.
 template <int NS, int NM, int NT = BLOCK>
-__device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
+__device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out, int64_t cs = 1) {
   __shared__ double sh[NT / 64][2 * NS + NM + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -44,19 +49,20 @@ __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out) {
     for (int q = 0; q < NS; ++q) {
       DD t{sh[0][2 * q], sh[0][2 * q + 1]};
       for (int w = 1; w < NT / 64; ++w) dd_add(t, DD{sh[w][2 * q], sh[w][2 * q + 1]});
-      out[2 * q] = t.hi;
-      out[2 * q + 1] = t.lo;
+      out[2 * q * cs] = t.hi;
+      out[(2 * q + 1) * cs] = t.lo;
     }
 #pragma unroll
     for (int q = 0; q < NM; ++q) {
       double t = sh[0][2 * NS + q];
       for (int w = 1; w < NT / 64; ++w) t = nan_min(t, sh[w][2 * NS + q]);
-      out[2 * NS + q] = t;
+      out[(2 * NS + q) * cs] = t;
     }
   }
 }
 
-// One workgroup folds all block partials (stride 2*NS+NM) in a fixed order; out gets NS sums
+// One workgroup folds all block partials (component q of block b at partials[q * nblocks + b]) in a fixed
+// order; out gets NS sums
 // (hi + lo) then NM minima, or, with INTERLEAVE (NS == NM), sum0, min0, sum1, min1, ...
 template <int NS, int NM, bool INTERLEAVE = false>
 __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ partials, int64_t nblocks,
@@ -77,7 +83,7 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
     for (int i = 0; i < BATCH; ++i) {
       const int64_t b = b0 + i * RB < nblocks ? b0 + i * RB : nblocks - 1;
 #pragma unroll
-      for (int q = 0; q < STRIDE; ++q) v[i][q] = partials[b * STRIDE + q];
+      for (int q = 0; q < STRIDE; ++q) v[i][q] = partials[q * nblocks + b];
     }
     // a block past the end leaves the accumulators as they are (selected, not branched, so that the compiler
     // keeps the eight loads ahead of the arithmetic)
@@ -110,6 +116,18 @@ template <int NS, int NM, bool INTERLEAVE = false>
 __global__ void __launch_bounds__(RB) reduce_partials_kernel(const double* __restrict__ partials, int64_t nblocks,
                                                                 double* __restrict__ out) {
   reduce_partials_body<NS, NM, INTERLEAVE>(partials, nblocks, out);
+}
+
+// The trial sweep's two reductions in one launch of two workgroups (soft terms, then the barrier's sums and
+// minima): each workgroup folds its array exactly as reduce_partials_kernel does, so the outputs are the same
+// bits, and the two folds run side by side instead of one after the other.
+__global__ void __launch_bounds__(RB) reduce_trial_partials_kernel(const double* __restrict__ soft,
+                                                                   const double* __restrict__ bar,
+                                                                   int64_t nblocks, double* __restrict__ out) {
+  if (blockIdx.x == 0)
+    reduce_partials_body<3, 0>(soft, nblocks, out);
+  else
+    reduce_partials_body<2, 2, true>(bar, nblocks, out + 3);
 }
 
 __device__ __forceinline__ void load_ih(const double* __restrict__ k, double ih[3]) {
@@ -204,7 +222,7 @@ __global__ void __launch_bounds__(BLOCK, 2) soft_energy_kernel(const double* __r
     dd_add(s[0], 0.5 * mu * cj);
     dd_add(s[1], 0.5 * nu * cc);
   }
-  block_reduce_store<3, 0>(s, none, partials + blk * 6);
+  block_reduce_store<3, 0>(s, none, partials + blk, cell_blocks(d));
 }
 
 // Doubles as unsigned keys in the same order (for atomic min/max).
@@ -387,7 +405,7 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_kernel(const double* __r
     }
     barrier_cell<WITH_BASE>(Eu, Eb, k, s, m);
   }
-  block_reduce_store<2, 2>(s, m, partials + blk * 6);
+  block_reduce_store<2, 2>(s, m, partials + blk, cell_blocks(d));
 }
 
 // The trial's soft and barrier energies in one sweep (the work of soft_energy_kernel and barrier_energy_kernel, in
@@ -494,8 +512,8 @@ __device__ __forceinline__ void trial_energy_block(const int64_t blk, const doub
     const double wm = wave_nan_min(m[0]);
     if ((threadIdx.x & 63) == 0 && wm < __builtin_inf()) atomicMin(umin, ordered_key(wm));   // NaN: no update
   }
-  block_reduce_store<3, 0>(s, none, soft_partials + blk * 6);
-  block_reduce_store<2, 2>(sb, m, barrier_partials + blk * 6);
+  block_reduce_store<3, 0>(s, none, soft_partials + blk, cell_blocks(d));
+  block_reduce_store<2, 2>(sb, m, barrier_partials + blk, cell_blocks(d));
 }
 
 template <bool WITH_BASE, bool BASE_IS_REF = false, bool QUIET = false>
@@ -645,7 +663,7 @@ __global__ void __launch_bounds__(BLOCK) soft_energy_f32_kernel(const double* __
     dd_add(s[1], 0.5 * (double)nu * ((double)cc.s - (double)cc.c));
     dd_add(s[2], (double)cd.s - (double)cd.c);
   }
-  block_reduce_store<3, 0>(s, none, partials + blk * 6);
+  block_reduce_store<3, 0>(s, none, partials + blk, cell_blocks(d));
 }
 
 template <bool WITH_BASE>
@@ -715,7 +733,7 @@ __global__ void __launch_bounds__(BLOCK) barrier_energy_f32_kernel(const double*
     m[0] = (double)m0;
     m[1] = (double)m1;
   }
-  block_reduce_store<2, 2>(s, m, partials + blk * 6);
+  block_reduce_store<2, 2>(s, m, partials + blk, cell_blocks(d));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1705,7 +1723,7 @@ __global__ void __launch_bounds__(BLOCK) node_energy_kernel(const double* __rest
       }
     }
   }
-  block_reduce_store<4, 0>(s, none, partials + (int64_t)blockIdx.x * 8);
+  block_reduce_store<4, 0>(s, none, partials + blockIdx.x, gridDim.x);
 }
 
 // The energy gradient on the free nodes in one pass: for interior node f (trapezoid weight 1, so q = omega) the
@@ -1896,8 +1914,7 @@ extern "C" int seqvp_q1_trial_energy_base_min(const double* u, const double* ref
   else
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
                        (const double*)nullptr, umin);
-  hipLaunchKernelGGL((reduce_partials_kernel<3, 0>), dim3(1), dim3(RB), 0, s, soft, nb, out);
-  hipLaunchKernelGGL((reduce_partials_kernel<2, 2, true>), dim3(1), dim3(RB), 0, s, bar, nb, out + 3);
+  hipLaunchKernelGGL(reduce_trial_partials_kernel, dim3(2), dim3(RB), 0, s, soft, bar, nb, out);
   return launch_status();
 }
 

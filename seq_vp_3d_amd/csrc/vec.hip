@@ -415,9 +415,11 @@ extern "C" int seqvp_direction_scalars(const double* theta, const double* theta_
   return launch_status();
 }
 
+namespace seqvp {
+
 // The first attempt's step from the direction scalars (`s` as seqvp_direction_scalars leaves them): the BB2
 // proposal with the reference's fallbacks (poisson_gradient.py:955-990), clipped to [step_min, step_max], then
-// min(proposal, 0.9 bound) (poisson_gradient.py:1340-1356). Written as the driver's host code computes it,
+// min(proposal, 0.9 bound) (poisson_gradient.py:1340-1356). Written as the solver's host code computes it,
 // operation for operation (IEEE sqrt and division, no contraction), so both give the same bits; min/max follow the
 // host's order of arguments for NaN (fmin/fmax return the number).
 __global__ void pg_step_kernel(const double* __restrict__ s, int has_prev, double prev_tau, double step_min,
@@ -466,6 +468,8 @@ __global__ void capped_step_kernel(const double* __restrict__ limit, const doubl
   }
   tau[0] = t;
 }
+
+}  // namespace seqvp
 
 extern "C" int seqvp_pg_step(const double* scalars, int has_prev, double prev_tau, double step_min, double step_max,
                              double initial_step, double* tau, void* stream) {

@@ -689,6 +689,7 @@ class PairState:
         _lib.call("seqvp_max_abs_diff", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(self.dot_out),
                   _lib.ptr(self.dot_ws), self.dot_ws.numel(), self._stream())
         top, bmax = self.dot_out[:2].tolist()
+        self.step_max_diff = top
         return self._ratio(top, bmax)
 
     def _ratio(self, top, bmax):

@@ -491,7 +491,9 @@ def solve_sequential(st, config=SequentialConfig(), inner_config=PGConfig(), bud
             resF = 16 * EPS * max(start["energy"].outer, st.energy.outer)
             total, fixed = (-dE, resE), (-dF, resF)
             cert = (-dE - resE) > 0 and (-dF - resF) > 0
-            same = bool(torch.equal(st.theta, start["theta"]))
+            # theta equals the start's iff max|theta - start| (left by step_ratio) is 0: theta is an accepted
+            # trial, whose energy (and so every entry) is finite, and -0.0 meets 0.0 in both tests
+            same = st.step_max_diff == 0.0
         accepted = bool(eligible and cert and not same)
         reason = ("ACCEPTED" if accepted else "INNER_FAILED_NOT_COMMITTED" if not eligible
                   else "NO_PROGRESS" if same else "DESCENT_CERTIFICATE_FAILED")

@@ -154,6 +154,7 @@ class CpuPairState:
     def step_ratio(self, a, b):
         a, b = a.numpy(), b.numpy()
         length = float(np.linalg.norm([ax[-1] - ax[0] for ax in self.grid.axes]))
+        self.step_max_diff = float(np.max(np.abs(a - b)))
         return float(np.max(np.abs(a - b)) / max(length, np.max(np.abs(b))))
 
     # trial -------------------------------------------------------------------------------------

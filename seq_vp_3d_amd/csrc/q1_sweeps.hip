@@ -64,20 +64,18 @@ __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out, i
 // One workgroup folds all block partials (component q of block b at partials[q * nblocks + b]) in a fixed
 // order; out gets NS sums
 // (hi + lo) then NM minima, or, with INTERLEAVE (NS == NM), sum0, min0, sum1, min1, ...
-template <int NS, int NM, bool INTERLEAVE = false>
-__device__ __forceinline__ void reduce_partials_body(const double* __restrict__ partials, int64_t nblocks,
-                                                     double* __restrict__ out) {
+// Fold of folder t (0 <= t < RB): blocks t, t + RB, t + 2 RB, ... in that order; eight of them are loaded at a
+// time (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop.
+template <int NS, int NM>
+__device__ __forceinline__ void fold_partials(const double* __restrict__ partials, int64_t nblocks, int64_t t, DD* s,
+                                              double* m) {
   constexpr int STRIDE = 2 * NS + NM;
-  DD s[NS];
-  double m[NM > 0 ? NM : 1];
 #pragma unroll
   for (int q = 0; q < NS; ++q) s[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = __builtin_inf();
-  // Each thread folds blocks t, t + RB, t + 2 RB, ... in that order; eight of them are loaded at a time
-  // (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop.
   constexpr int BATCH = 8;
-  for (int64_t b0 = threadIdx.x; b0 < nblocks; b0 += BATCH * RB) {
+  for (int64_t b0 = t; b0 < nblocks; b0 += BATCH * RB) {
     double v[BATCH][STRIDE];
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
@@ -101,6 +99,15 @@ __device__ __forceinline__ void reduce_partials_body(const double* __restrict__ 
       for (int q = 0; q < NM; ++q) m[q] = in ? nan_min(m[q], v[i][2 * NS + q]) : m[q];
     }
   }
+}
+
+template <int NS, int NM, bool INTERLEAVE = false>
+__device__ __forceinline__ void reduce_partials_body(const double* __restrict__ partials, int64_t nblocks,
+                                                     double* __restrict__ out) {
+  constexpr int STRIDE = 2 * NS + NM;
+  DD s[NS];
+  double m[NM > 0 ? NM : 1];
+  fold_partials<NS, NM>(partials, nblocks, threadIdx.x, s, m);
   __shared__ double fin[STRIDE];
   block_reduce_store<NS, NM, RB>(s, m, fin);
   __syncthreads();
@@ -118,16 +125,67 @@ __global__ void __launch_bounds__(RB) reduce_partials_kernel(const double* __res
   reduce_partials_body<NS, NM, INTERLEAVE>(partials, nblocks, out);
 }
 
-// The trial sweep's two reductions in one launch of two workgroups (soft terms, then the barrier's sums and
-// minima): each workgroup folds its array exactly as reduce_partials_kernel does, so the outputs are the same
-// bits, and the two folds run side by side instead of one after the other.
-__global__ void __launch_bounds__(RB) reduce_trial_partials_kernel(const double* __restrict__ soft,
-                                                                   const double* __restrict__ bar,
-                                                                   int64_t nblocks, double* __restrict__ out) {
-  if (blockIdx.x == 0)
-    reduce_partials_body<3, 0>(soft, nblocks, out);
+// The trial sweep's two reductions (soft terms; the barrier's sums and minima), in two short launches that give the
+// bits of reduce_partials_kernel: the RB folders of its workgroup run as 4 workgroups of RB / 4 lanes per array
+// (grid (4, 2)), each wave adds its lanes as block_reduce_store does and stores its sum (stage: 2 arrays x RB / 64
+// waves x 6 doubles), and one thread per array then adds the RB / 64 wave sums in wave order. The folds then run on
+// eight CUs instead of one.
+constexpr int TRIAL_STAGE_WAVES = RB / 64;
+constexpr int64_t TRIAL_STAGE_DOUBLES = 2 * TRIAL_STAGE_WAVES * 6;
+// byte offset of the stage in the trial's workspace: after the two partial arrays (nb x 6 doubles each) and the
+// 256-byte slot of the running least J
+inline size_t trial_stage_offset(int64_t nb) { return (size_t)nb * 12 * sizeof(double) + 256; }
+
+template <int NS, int NM>
+__device__ __forceinline__ void trial_stage_fold(const double* __restrict__ partials, int64_t nblocks,
+                                                 double* __restrict__ stage) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  DD s[NS];
+  double m[NM > 0 ? NM : 1];
+  fold_partials<NS, NM>(partials, nblocks, t, s, m);
+#pragma unroll
+  for (int q = 0; q < NS; ++q) s[q] = dd_wave_sum(s[q]);
+#pragma unroll
+  for (int q = 0; q < NM; ++q) m[q] = wave_nan_min(m[q]);
+  if ((threadIdx.x & 63) == 0) {
+    double* w = stage + (t >> 6) * 6;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) { w[2 * q] = s[q].hi; w[2 * q + 1] = s[q].lo; }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) w[2 * NS + q] = m[q];
+  }
+}
+
+__global__ void __launch_bounds__(RB / 4) reduce_trial_stage_kernel(const double* __restrict__ soft,
+                                                                    const double* __restrict__ bar, int64_t nblocks,
+                                                                    double* __restrict__ stage) {
+  if (blockIdx.y == 0)
+    trial_stage_fold<3, 0>(soft, nblocks, stage);
   else
-    reduce_partials_body<2, 2, true>(bar, nblocks, out + 3);
+    trial_stage_fold<2, 2>(bar, nblocks, stage + TRIAL_STAGE_WAVES * 6);
+}
+
+__global__ void reduce_trial_final_kernel(const double* __restrict__ stage, double* __restrict__ out) {
+  if (threadIdx.x == 0) {           // soft: three sums
+    const double* w = stage;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      DD t{w[2 * q], w[2 * q + 1]};
+      for (int v = 1; v < TRIAL_STAGE_WAVES; ++v) dd_add(t, DD{w[v * 6 + 2 * q], w[v * 6 + 2 * q + 1]});
+      out[q] = t.hi + t.lo;
+    }
+  } else if (threadIdx.x == 1) {    // barrier: sum, min, sum, min
+    const double* w = stage + TRIAL_STAGE_WAVES * 6;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      DD t{w[2 * q], w[2 * q + 1]};
+      for (int v = 1; v < TRIAL_STAGE_WAVES; ++v) dd_add(t, DD{w[v * 6 + 2 * q], w[v * 6 + 2 * q + 1]});
+      out[3 + 2 * q] = t.hi + t.lo;
+      double mm = w[4 + q];
+      for (int v = 1; v < TRIAL_STAGE_WAVES; ++v) mm = nan_min(mm, w[v * 6 + 4 + q]);
+      out[3 + 2 * q + 1] = mm;
+    }
+  }
 }
 
 __device__ __forceinline__ void load_ih(const double* __restrict__ k, double ih[3]) {
@@ -1861,7 +1919,7 @@ static size_t q1_workspace(int op, int64_t nx, int64_t ny, int64_t nz) {
     case 5: return root_workspace(d).bytes;
     case 8: return (size_t)d.ncell * sizeof(double);
     case 9: return (size_t)1024 * 8 * sizeof(double);
-    case 10: return (size_t)nblocks * 12 * sizeof(double) + 256;   // + the running least J of the sweep
+    case 10: return trial_stage_offset(nblocks) + TRIAL_STAGE_DOUBLES * sizeof(double);   // partials, least J, stage
     default: return 0;
   }
 }
@@ -1914,7 +1972,9 @@ extern "C" int seqvp_q1_trial_energy_base_min(const double* u, const double* ref
   else
     hipLaunchKernelGGL((trial_energy_kernel<false>), dim3(nb), dim3(BLOCK), 0, s, u, ref, ref, k, d, soft, bar,
                        (const double*)nullptr, umin);
-  hipLaunchKernelGGL(reduce_trial_partials_kernel, dim3(2), dim3(RB), 0, s, soft, bar, nb, out);
+  double* stage = (double*)((char*)ws + trial_stage_offset(nb));
+  hipLaunchKernelGGL(reduce_trial_stage_kernel, dim3(4, 2), dim3(RB / 4), 0, s, soft, bar, nb, stage);
+  hipLaunchKernelGGL(reduce_trial_final_kernel, dim3(1), dim3(64), 0, s, stage, out);
   return launch_status();
 }
 

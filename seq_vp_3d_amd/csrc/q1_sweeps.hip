@@ -166,8 +166,12 @@ __global__ void __launch_bounds__(RB / 4) reduce_trial_stage_kernel(const double
 }
 
 __global__ void reduce_trial_final_kernel(const double* __restrict__ stage, double* __restrict__ out) {
+  // the wave sums come to LDS first (one load per lane), so the serial sums below wait on no global load
+  __shared__ double st[TRIAL_STAGE_DOUBLES];
+  for (int i = threadIdx.x; i < TRIAL_STAGE_DOUBLES; i += blockDim.x) st[i] = stage[i];
+  __syncthreads();
   if (threadIdx.x == 0) {           // soft: three sums
-    const double* w = stage;
+    const double* w = st;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       DD t{w[2 * q], w[2 * q + 1]};
@@ -175,7 +179,7 @@ __global__ void reduce_trial_final_kernel(const double* __restrict__ stage, doub
       out[q] = t.hi + t.lo;
     }
   } else if (threadIdx.x == 1) {    // barrier: sum, min, sum, min
-    const double* w = stage + TRIAL_STAGE_WAVES * 6;
+    const double* w = st + TRIAL_STAGE_WAVES * 6;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       DD t{w[2 * q], w[2 * q + 1]};

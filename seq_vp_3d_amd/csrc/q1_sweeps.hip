@@ -64,9 +64,10 @@ __device__ void block_reduce_store(DD* s, double* m, double* __restrict__ out, i
 // One workgroup folds all block partials (component q of block b at partials[q * nblocks + b]) in a fixed
 // order; out gets NS sums
 // (hi + lo) then NM minima, or, with INTERLEAVE (NS == NM), sum0, min0, sum1, min1, ...
-// Fold of folder t (0 <= t < RB): blocks t, t + RB, t + 2 RB, ... in that order; eight of them are loaded at a
-// time (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop.
-template <int NS, int NM>
+// Fold of folder t (0 <= t < RB): blocks t, t + RB, t + 2 RB, ... in that order; BATCH of them are loaded at a
+// time (unconditionally, from clamped addresses) so their latencies overlap instead of serialising the loop. The
+// batch size changes only how many loads are in flight, not the order of the additions.
+template <int NS, int NM, int BATCH = 8>
 __device__ __forceinline__ void fold_partials(const double* __restrict__ partials, int64_t nblocks, int64_t t, DD* s,
                                               double* m) {
   constexpr int STRIDE = 2 * NS + NM;
@@ -74,7 +75,6 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ partial
   for (int q = 0; q < NS; ++q) s[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = __builtin_inf();
-  constexpr int BATCH = 8;
   for (int64_t b0 = t; b0 < nblocks; b0 += BATCH * RB) {
     double v[BATCH][STRIDE];
 #pragma unroll
@@ -83,8 +83,11 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ partial
 #pragma unroll
       for (int q = 0; q < STRIDE; ++q) v[i][q] = partials[q * nblocks + b];
     }
-    // a block past the end leaves the accumulators as they are (selected, not branched, so that the compiler
-    // keeps the eight loads ahead of the arithmetic)
+    // with a deep batch (one wave per SIMD, a large register budget) every load of the batch is issued before the
+    // first addition: the scheduler would otherwise sink each load to its use and wait on them a few at a time. At
+    // RB lanes per workgroup the budget is 128 VGPRs, and the barrier would spill.
+    if constexpr (BATCH > 8) __builtin_amdgcn_sched_barrier(0);
+    // a block past the end leaves the accumulators as they are (selected, not branched)
 #pragma unroll
     for (int i = 0; i < BATCH; ++i) {
       const bool in = b0 + i * RB < nblocks;
@@ -142,7 +145,9 @@ __device__ __forceinline__ void trial_stage_fold(const double* __restrict__ part
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   DD s[NS];
   double m[NM > 0 ? NM : 1];
-  fold_partials<NS, NM>(partials, nblocks, t, s, m);
+  // one wave per SIMD here: the register file holds 32 blocks' partials at once, so at the native size (26 blocks
+  // per folder) every load of a folder is in flight together
+  fold_partials<NS, NM, 32>(partials, nblocks, t, s, m);
 #pragma unroll
   for (int q = 0; q < NS; ++q) s[q] = dd_wave_sum(s[q]);
 #pragma unroll
@@ -156,7 +161,7 @@ __device__ __forceinline__ void trial_stage_fold(const double* __restrict__ part
   }
 }
 
-__global__ void __launch_bounds__(RB / 4) reduce_trial_stage_kernel(const double* __restrict__ soft,
+__global__ void __launch_bounds__(RB / 4) __attribute__((amdgpu_waves_per_eu(1, 1))) reduce_trial_stage_kernel(const double* __restrict__ soft,
                                                                     const double* __restrict__ bar, int64_t nblocks,
                                                                     double* __restrict__ stage) {
   if (blockIdx.y == 0)

@@ -48,7 +48,8 @@ int seqvp_q1_soft_energy(const double* u, const double* ref, const double* base,
 /* The trial's soft and barrier energies in one sweep (replaces the pair FusedQ1.soft_energy + barrier_energy of
  * AcceleratedHybridBackend._energy, accelerated.py:1574-1664): out[0:3] = seqvp_q1_soft_energy's [jd, curl,
  * difference], out[3:7] = seqvp_q1_barrier_energy's [value, min J, unweighted change, min base J], bit for bit.
- * `out` is 7 device doubles; workspace op 10 (12 doubles per 256-cell block, 256 bytes and 192 doubles of reduction stage). */
+ * `out` is 7 device doubles; workspace op 10 (12 doubles per 256-cell block, 256 bytes, and 192 doubles for the
+ * reduction's stage). */
 int seqvp_q1_trial_energy(const double* u, const double* ref, const double* base, const double* k,
                           int64_t nx, int64_t ny, int64_t nz, double* out, void* workspace,
                           size_t workspace_bytes, void* stream);

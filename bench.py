@@ -58,15 +58,15 @@ TRAFFIC_BYTES_NATIVE = {
     "seqvp_q1_soft_energy": 986e6,
     "seqvp_q1_barrier_energy": 362e6,
     "seqvp_q1_trial_energy": 988e6,
-    "seqvp_q1_trial_energy_base_min": 988e6,
+    "seqvp_q1_trial_energy_base_min": 991e6,
     "seqvp_q1_gradient": 3666e6,
     "seqvp_q1_gradient_free": 3940e6,
     "seqvp_q1_joint_free": 5973e6,
-    "seqvp_jd_root": 381e6,
-    "seqvp_jd_root_free": 381e6,
+    "seqvp_jd_root": 380e6,
+    "seqvp_jd_root_free": 380e6,
     "seqvp_certificate_screen": 283e6,
     "seqvp_spline_warp": 610e6,
-    "seqvp_dst_poisson": 2533e6,
+    "seqvp_dst_poisson": 2531e6,
 }
 
 
